@@ -1,0 +1,168 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+* golden vectors written by the reference itself (tests/golden/) -- kernels
+  bit-exact, tables bit-exact, full operator to FFT-vs-direct rounding;
+* closed forms: analytic traveltime (pytests/test_kirchhoff.py:139-157),
+  single-point impulse response, convolution against numpy.convolve;
+* brute force: the dense matrix of the forward equals the transpose of the
+  dense matrix of the adjoint;
+* invariants: single table == separable tables (test_kirchhoff.py:320-462);
+* library routines: LSQR against scipy.sparse.linalg.lsqr, CGLS against
+  numpy.linalg.lstsq (pytests/test_solver.py:170-232).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import geometry as G
+
+
+def _load(name):
+    return np.load(os.path.join(G.GOLDEN, name + ".npz"))
+
+
+@pytest.mark.parametrize("name", sorted(G.golden_cases()))
+def test_oracle_vs_golden(name):
+    g = _load(name)
+    Op = O.KirchhoffOracle(**G.golden_kwargs(name, g))
+    for k in ["trav_srcs", "trav_recs", "amp_srcs", "amp_recs", "angle_srcs",
+              "angle_recs"]:
+        if k in g.files:
+            np.testing.assert_array_equal(getattr(Op, k), g[k], err_msg=k)
+    if "spread" in g.files:
+        # the engine kernels reproduce the reference's float64 arithmetic and
+        # per-`+=` float32 rounding exactly
+        np.testing.assert_array_equal(Op.spread(g["m"]), g["spread"])
+        np.testing.assert_array_equal(Op.gather(g["dk"], in_f64=True), g["gather"])
+    tol = 1e-6 if Op.dtype == np.float32 else 1e-13
+    assert G.relerr(Op.matvec(g["m"]), g["fwd"]) < tol
+    assert G.relerr(Op.rmatvec(g["d"]), g["adj"]) < tol
+
+
+def test_traveltime_analytic_closed_form():
+    """pytests/test_kirchhoff.py:139-157."""
+    src = np.array([100, 0])[:, np.newaxis]
+    ts, tr, ds, dr, _, _ = O.analytic_tables(np.arange(0, 200, 1), np.arange(0, 200, 1),
+                                             src, src, G.v0)
+    assert ds[0, 0] + dr[0, 0] == 200
+    assert ts[0, 0] == 100 / G.v0
+    assert tr[0, 0] == 100 / G.v0
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_impulse_response_closed_form(dtype):
+    """A unit reflector at one image point spreads into two taps per trace at
+    (t_s + t_r)/dt with linear-interpolation weights."""
+    Op = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                           mode="analytic", dtype=dtype)
+    ix, iz = 5, 13
+    ii = ix * G.PAR["nz"] + iz
+    m = np.zeros(Op.ni)
+    m[ii] = 1.0
+    y = Op.spread(m).reshape(Op.ns * Op.nr, Op.nt)
+    px, pz = G.x[ix], G.z[iz]
+    for isrc in range(Op.ns):
+        for irec in range(Op.nr):
+            T = (np.hypot(px - G.s2d[0, isrc], pz - G.s2d[1, isrc])
+                 + np.hypot(px - G.r2d[0, irec], pz - G.r2d[1, irec])) / G.v0
+            q = T / G.PAR["dt"]
+            it = int(q)
+            f = q - it
+            expect = np.zeros(Op.nt)
+            if 0 <= it < Op.nt - 1:
+                expect[it], expect[it + 1] = 1 - f, f
+            np.testing.assert_allclose(y[isrc * Op.nr + irec], expect, atol=1e-7)
+
+
+def test_conv_same_vs_numpy_convolve():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((3, 57))
+    for nh in (1, 5, 21, 41):
+        h = rng.standard_normal(nh)
+        full = np.stack([np.convolve(r, h) for r in x])
+        c = (nh - 1) // 2
+        np.testing.assert_allclose(O.conv_same(x, h), full[:, c:c + 57], atol=1e-12)
+
+
+@pytest.mark.parametrize("nh,offset", [(21, 10), (21, 3), (21, 17), (20, 4), (20, 12)])
+def test_filter_offset_centres_wavelet(nh, offset):
+    """After Convolve1D's padding (convolve1d.py:72-86), an impulse at n0
+    returns h with h[offset] at n0."""
+    rng = np.random.default_rng(1)
+    h = rng.standard_normal(nh)
+    hp = O.padded_filter(h, offset)
+    assert hp.size % 2 == 1
+    n0 = 40
+    x = np.zeros((1, 90))
+    x[0, n0] = 1.0
+    y = O.conv_same(x, hp)[0]
+    np.testing.assert_allclose(y[n0 - offset:n0 - offset + nh], h, atol=1e-14)
+    assert abs(y).sum() == pytest.approx(abs(h).sum())
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_dense_forward_is_transpose_of_adjoint(dynamic):
+    Op = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                           mode="analytic", dynamic=dynamic, angleaperture=50)
+    n, m = Op.shape[1], Op.shape[0]
+    A = np.stack([Op.matvec(e) for e in np.eye(n)], axis=1)
+    sel = np.random.default_rng(2).choice(m, 60, replace=False)
+    AT = np.stack([Op.rmatvec(np.eye(m)[i]) for i in sel], axis=0)
+    np.testing.assert_allclose(AT, A[sel], atol=1e-12 * np.abs(A).max())
+
+
+def test_single_table_equals_separable():
+    Op = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                           mode="analytic")
+    trav = (Op.trav_srcs[:, :, None] + Op.trav_recs[:, None, :]).reshape(Op.ni, -1)
+    Op1 = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                            mode="byot", trav=trav)
+    rng = np.random.default_rng(3)
+    m = rng.standard_normal(Op.ni)
+    d = rng.standard_normal(Op.shape[0])
+    np.testing.assert_allclose(Op.matvec(m), Op1.matvec(m), atol=1e-12)
+    np.testing.assert_allclose(Op.rmatvec(d), Op1.rmatvec(d), atol=1e-12)
+
+
+def test_solvers_vs_golden():
+    g = _load("solvers_dyn2d_f64")
+    Op = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                           mode="analytic", dynamic=True)
+    y = g["dobs"]
+    x, istop, iiter, r1, r2, cost = O.cgls(Op, y, niter=10, tol=0.0)
+    np.testing.assert_allclose(x, g["cgls_x"].ravel(), rtol=1e-9, atol=1e-12)
+    assert (istop, iiter) == (int(g["cgls_istop"]), int(g["cgls_iiter"]))
+    np.testing.assert_allclose([r1, r2], [g["cgls_r1"], g["cgls_r2"]], rtol=1e-9)
+    np.testing.assert_allclose(cost, g["cgls_cost"], rtol=1e-9)
+    x, istop, iiter, r1, r2, cost = O.cgls(Op, y, x0=g["x0"], niter=8, damp=0.3, tol=0.0)
+    np.testing.assert_allclose(x, g["cglsd_x"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose([r1, r2], [g["cglsd_r1"], g["cglsd_r2"]], rtol=1e-9)
+    np.testing.assert_allclose(cost, g["cglsd_cost"], rtol=1e-9)
+    for pre, kw in (("lsqr_", dict(niter=10)),
+                    ("lsqrd_", dict(x0=g["x0"], damp=0.2, niter=7))):
+        res = O.lsqr(Op, y, calc_var=True, **kw)
+        names = ["x", "istop", "iiter", "r1", "r2", "anorm", "acond", "arnorm",
+                 "xnorm", "var", "cost"]
+        for k, v in zip(names, res):
+            np.testing.assert_allclose(np.ravel(v), np.ravel(g[pre + k]), rtol=1e-8,
+                                       atol=1e-12, err_msg=pre + k)
+
+
+@pytest.mark.parametrize("shape", [(11, 11), (31, 11)])
+def test_solvers_vs_library(shape):
+    """pytests/test_solver.py:170-232 restated: x = ones is recovered; LSQR
+    matches scipy.sparse.linalg.lsqr."""
+    from scipy.sparse.linalg import lsqr as sp_lsqr
+    rng = np.random.default_rng(10)
+    A = rng.normal(0, 10, shape) + 10 * np.eye(*shape)
+    Op = O.DenseOp(A)
+    xt = np.ones(shape[1])
+    y = A @ xt
+    x = O.cgls(Op, y, niter=2 * shape[1], tol=0.0)[0]
+    np.testing.assert_allclose(x, np.linalg.lstsq(A, y, rcond=None)[0], rtol=1e-6)
+    res = O.lsqr(Op, y, niter=shape[1], atol=0, btol=0, conlim=0)
+    ref = sp_lsqr(A, y, iter_lim=shape[1], atol=0, btol=0, conlim=0)
+    np.testing.assert_allclose(res[0], ref[0], rtol=1e-6)
+    np.testing.assert_allclose(res[0], xt, rtol=1e-4)
