@@ -1,0 +1,136 @@
+/*
+ * kirchx.h -- C ABI of the MI355X Kirchhoff demigration/migration library
+ * (libkirchx.so).  Plain C types only: device pointers are `void *`,
+ * streams are `hipStream_t` passed as `void *`, sizes are int32/int64.
+ *
+ * Every entry point is stream-ordered and asynchronous: it enqueues kernels
+ * on `stream` and returns.  Return value 0 means success; a negative value is
+ * an argument error (KX_EINVAL ...) and a positive value is a hipError_t.
+ * kx_last_error() gives a message for the calling thread's last failure.
+ * The library allocates no device memory; scratch space is a caller-owned
+ * workspace whose size kx_workspace_bytes() reports.
+ *
+ * Reference interfaces replaced (PyLops, pylops/waveeqprocessing/kirchhoff.py
+ * = K, pylops/signalprocessing/convolve1d.py = C1D,
+ * pylops/optimization/cls_basic.py = CB):
+ *   kx_spread   <- Kirchhoff._travsrcrec_kirch_matvec   K:727-752
+ *                  Kirchhoff._ampsrcrec_kirch_matvec    K:783-889
+ *                  Kirchhoff._trav_kirch_matvec         K:683-702
+ *   kx_gather   <- Kirchhoff._travsrcrec_kirch_rmatvec  K:754-781
+ *                  Kirchhoff._ampsrcrec_kirch_rmatvec   K:891-1001
+ *                  Kirchhoff._trav_kirch_rmatvec        K:704-725
+ *   kx_conv     <- _Convolve1Dshort._matvec/_rmatvec    C1D:99-115
+ *                  (scipy.signal.fftconvolve(mode="same"))
+ *   kx_forward  <- Kirchhoff._matvec                    K:1048-1094
+ *   kx_adjoint  <- Kirchhoff._rmatvec                   K:1096-1143
+ *   kx_encode_table / kx_tile_table
+ *               <- the table setup of Kirchhoff.__init__ K:304-398
+ *                  (device layout only; values come from the caller)
+ *   kx_dot, kx_cgls_*, kx_lsqr_* <- the vector algebra of CGLS.step
+ *                  CB:510-564 and LSQR.step CB:987-1136
+ */
+#ifndef KIRCHX_H
+#define KIRCHX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KX_ABI_VERSION 1
+
+/* element types */
+#define KX_F32 0
+#define KX_F64 1
+#define KX_F16 2 /* table offsets only */
+
+/* error codes (negative: argument errors) */
+#define KX_OK 0
+#define KX_EINVAL -1
+#define KX_ESHAPE -2
+#define KX_ETYPE -3
+#define KX_EABI -4
+#define KX_ELDS -5
+
+/* Operator plan.  All table pointers are device pointers in the tiled
+ * layout produced by kx_encode_table / kx_tile_table:
+ *   offsets  off[n][ntiles][64]   element type tab_fmt (f32, f16 or f64),
+ *            traveltime in samples minus base,
+ *   bases    base[n][ntiles]      int32, samples,
+ *   weights  wp/wq[n][ntiles][64] element type dtype (dynamic only).
+ * A tile is a tz*tx*ty = 64 block of image points; tiles are enumerated
+ * z fastest, then x, then y; lanes inside a tile likewise. */
+typedef struct kx_plan {
+    int32_t abi_version; /* KX_ABI_VERSION */
+    int32_t dtype;       /* KX_F32 or KX_F64: model, data and accumulation */
+    int32_t tab_fmt;     /* KX_F32, KX_F16 (dtype F32 only) or KX_F64 */
+    int32_t dynamic;     /* 1: amplitude/obliquity/aperture weights */
+    int32_t ns, nr, nt;  /* sources, receivers, time samples */
+    int32_t nx, ny, nz;  /* image grid, ny = 1 in 2D */
+    int32_t tz, tx, ty;  /* tile shape, tz*tx*ty == 64 */
+    int32_t ntz, ntx, nty;
+    int64_t ntiles;      /* ntz*ntx*nty */
+    int32_t neg_trav;    /* 1 if a traveltime may be negative (truncation path) */
+    int32_t rblock;      /* receivers per spread workgroup, 0 = automatic */
+    int32_t nh;          /* wavelet length after offset padding (odd) */
+    int32_t pad0;
+    const void *off_s;
+    const int32_t *base_s;
+    const void *off_r;
+    const int32_t *base_r;
+    const void *wp_s, *wq_s, *wp_r, *wq_r; /* dynamic weights or NULL */
+    const int32_t *izmin;                  /* [ns*nr] aperture gate or NULL */
+    const void *h;                         /* wavelet, nh elements of dtype */
+    const void *hstar;                     /* flipped wavelet */
+} kx_plan;
+
+int kx_version(void);
+const char *kx_last_error(void);
+
+/* Bytes of workspace kx_forward / kx_adjoint need (one [ns*nr][nt] trace
+ * buffer of dtype). */
+int64_t kx_workspace_bytes(const kx_plan *p);
+
+/* Table layout: reads a [n][ni] float64 table tabT (reference table
+ * transposed), multiplies by `scale` (1/dt), and writes off/base in the
+ * tiled layout; fmt = KX_F32 / KX_F16 / KX_F64.  Padding lanes replicate
+ * the nearest grid point. */
+int kx_encode_table(const kx_plan *geom, const double *tabT, int32_t n,
+                    double scale, int32_t fmt, void *off, int32_t *base,
+                    void *stream);
+
+/* Reorders a [n][ni] float64 table into [n][ntiles][64] of dtype `fmt`
+ * (KX_F32 or KX_F64), zero padded (dynamic weights). */
+int kx_tile_table(const kx_plan *geom, const double *tabT, int32_t n,
+                  int32_t fmt, void *out, void *stream);
+
+/* The engine kernels alone.
+ * kx_spread: traces[ns*nr][nt] = sum over image points of m spread at the
+ *            two-sample linear interpolation of t_s + t_r (overwrites).
+ * kx_gather: m[ni] = sum over traces of the interpolated trace samples
+ *            (overwrites). */
+int kx_spread(const kx_plan *p, const void *m, void *traces, void *stream);
+int kx_gather(const kx_plan *p, const void *traces, void *m, void *stream);
+
+/* "same"-mode convolution of each of ntr rows of length nt with an odd
+ * filter h[nh]:  y[n] = sum_k h[k] x[n + (nh-1)/2 - k].  x != y. */
+int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const void *h,
+            int32_t nh, int32_t dtype, void *stream);
+
+/* Full operator: forward = spread then convolve with h; adjoint =
+ * correlate (convolve with hstar) then gather.  work: kx_workspace_bytes. */
+int kx_forward(const kx_plan *p, const void *m, void *d, void *work, void *stream);
+int kx_adjoint(const kx_plan *p, const void *d, void *m, void *work, void *stream);
+
+/* ---- vector algebra for the solvers (float64 accumulation) ---- */
+/* Bytes of workspace for the reductions below. */
+int64_t kx_reduce_workspace_bytes(void);
+/* out[0] = sum a[i]*b[i] accumulated in float64; out is a device double. */
+int kx_dot(const void *a, const void *b, int64_t n, int32_t dtype, double *out,
+           void *work, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

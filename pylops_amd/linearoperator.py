@@ -1,0 +1,312 @@
+"""LinearOperator: the operator interface of the reference
+(pylops/linearoperator.py), restricted to what the Kirchhoff / solver path
+uses: shapes and dims, matvec/rmatvec/matmat/rmatmat with their shape checks
+and counters, ``dot`` / ``@`` / ``*`` with dims-shaped ndarray support,
+``.H`` / ``.T``, scaling, sums and products.  Works on numpy arrays and on
+torch tensors (device or host)."""
+import numpy as np
+
+from pylops_amd.config import get_ndarray_multiplication
+from pylops_amd.utils.backend import is_torch, vstack
+from pylops_amd.utils.decorators import count
+
+__all__ = ["LinearOperator", "aslinearoperator"]
+
+
+def _isscalar(x):
+    return np.isscalar(x) or (is_torch(x) and x.ndim == 0)
+
+
+class LinearOperator:
+    """Generic linear operator (LO:52-1563, subset).
+
+    Parameters follow the reference: ``Op`` (wrapped operator), ``dtype``,
+    ``shape``, ``dims``, ``dimsd``, ``clinear``, ``explicit``,
+    ``forceflat``, ``name``."""
+
+    def __init__(self, Op=None, dtype=None, shape=None, dims=None, dimsd=None,
+                 clinear=None, explicit=None, forceflat=None, name=None):
+        self.Op = Op
+        if Op is not None:
+            dtype = Op.dtype if dtype is None else dtype
+            shape = Op.shape if shape is None else shape
+            dims = getattr(Op, "dims", (Op.shape[1],)) if dims is None else dims
+            dimsd = getattr(Op, "dimsd", (Op.shape[0],)) if dimsd is None else dimsd
+            clinear = getattr(Op, "clinear", True) if clinear is None else clinear
+            explicit = getattr(Op, "explicit", False) if explicit is None else explicit
+            forceflat = getattr(Op, "forceflat", None) if forceflat is None else forceflat
+            name = getattr(Op, "name", None) if name is None else name
+        self.dtype = np.dtype(dtype) if dtype is not None else None
+        self._dims = tuple(int(d) for d in dims) if dims is not None else None
+        self._dimsd = tuple(int(d) for d in dimsd) if dimsd is not None else None
+        if shape is not None:
+            self._shape = (int(shape[0]), int(shape[1]))
+        elif self._dims is not None and self._dimsd is not None:
+            self._shape = (int(np.prod(self._dimsd)), int(np.prod(self._dims)))
+        else:
+            self._shape = None
+        self.clinear = True if clinear is None else clinear
+        self.explicit = False if explicit is None else explicit
+        self.forceflat = forceflat
+        self.name = name
+        self.matvec_count = 0
+        self.rmatvec_count = 0
+        self.matmat_count = 0
+        self.rmatmat_count = 0
+
+    # ---- shape bookkeeping (LO:158-288) ----
+    @property
+    def shape(self):
+        if self._shape is None:
+            raise AttributeError(
+                f"'{type(self).__name__}' object has no attribute 'shape' nor both "
+                "fallback attributes ('dims', 'dimsd')")
+        return self._shape
+
+    @shape.setter
+    def shape(self, new):
+        self._shape = (int(new[0]), int(new[1]))
+
+    @property
+    def dims(self):
+        return self._dims if self._dims is not None else (self.shape[1],)
+
+    @dims.setter
+    def dims(self, new):
+        self._dims = tuple(int(d) for d in np.atleast_1d(new))
+
+    @property
+    def dimsd(self):
+        return self._dimsd if self._dimsd is not None else (self.shape[0],)
+
+    @dimsd.setter
+    def dimsd(self, new):
+        self._dimsd = tuple(int(d) for d in np.atleast_1d(new))
+
+    def __repr__(self):
+        M, N = self.shape
+        return f"<{M}x{N} {type(self).__name__} with dtype={self.dtype}>"
+
+    def _copy_attributes(self, dest, exclude=()):
+        for attr in ("dims", "dimsd", "clinear", "explicit", "forceflat", "name"):
+            if attr not in exclude:
+                setattr(dest, attr, getattr(self, attr))
+
+    # ---- handlers ----
+    def _matvec(self, x):
+        if self.Op is not None:
+            return self.Op._matvec(x)
+        raise NotImplementedError
+
+    def _rmatvec(self, x):
+        if self.Op is not None:
+            return self.Op._rmatvec(x)
+        raise NotImplementedError
+
+    def _matmat(self, X):
+        return vstack([self.matvec(col.reshape(-1)) for col in X.T]).T
+
+    def _rmatmat(self, X):
+        return vstack([self.rmatvec(col.reshape(-1)) for col in X.T]).T
+
+    # ---- public products (LO:488-702) ----
+    @count(forward=True)
+    def matvec(self, x):
+        M, N = self.shape
+        if tuple(x.shape) != (N,) and tuple(x.shape) != (N, 1):
+            raise ValueError(
+                f"Dimension mismatch. Got {tuple(x.shape)}, but expected ({N},) or ({N}, 1).")
+        y = self._matvec(x)
+        if x.ndim == 1:
+            return y.reshape(M)
+        return y.reshape(M, 1)
+
+    @count(forward=False)
+    def rmatvec(self, x):
+        M, N = self.shape
+        if tuple(x.shape) != (M,) and tuple(x.shape) != (M, 1):
+            raise ValueError(
+                f"Dimension mismatch. Got {tuple(x.shape)}, but expected ({M},) or ({M}, 1).")
+        y = self._rmatvec(x)
+        if x.ndim == 1:
+            return y.reshape(N)
+        return y.reshape(N, 1)
+
+    @count(forward=True, matmat=True)
+    def matmat(self, X):
+        if X.ndim != 2:
+            raise ValueError(f"Expected 2-d ndarray or matrix, not {X.ndim}-d ndarray")
+        if X.shape[0] != self.shape[1]:
+            raise ValueError(f"Dimension mismatch: {self.shape}, {tuple(X.shape)}")
+        return self._matmat(X)
+
+    @count(forward=False, matmat=True)
+    def rmatmat(self, X):
+        if X.ndim != 2:
+            raise ValueError(f"Expected 2-d ndarray or matrix, not {X.ndim}-d ndarray")
+        if X.shape[0] != self.shape[0]:
+            raise ValueError(f"Dimension mismatch: {self.shape}, {tuple(X.shape)}")
+        return self._rmatmat(X)
+
+    def dot(self, x):
+        if isinstance(x, LinearOperator):
+            return _ProductLinearOperator(self, x)
+        if _isscalar(x):
+            return _ScaledLinearOperator(self, x)
+        if not get_ndarray_multiplication() and (
+                x.ndim > 2 or (x.ndim == 2 and x.shape[0] != self.shape[1])):
+            raise ValueError(
+                "Operator can only be applied 1D vectors or 2D matrices. "
+                "Enable ndarray multiplication with pylops.set_ndarray_multiplication(True).")
+        xshape = tuple(x.shape)
+        is_dims_shaped = xshape == tuple(self.dims)
+        is_dims_shaped_matrix = len(xshape) > 1 and xshape[:-1] == tuple(self.dims)
+        if is_dims_shaped:
+            x = x.ravel()
+        if is_dims_shaped_matrix and not self.forceflat:
+            x = x.reshape((-1, xshape[-1]))
+        if x.ndim == 1:
+            y = self.matvec(x)
+            if is_dims_shaped and not self.forceflat and get_ndarray_multiplication():
+                y = y.reshape(self.dimsd)
+            return y
+        if x.ndim == 2:
+            y = self.matmat(x)
+            if is_dims_shaped_matrix and not self.forceflat and get_ndarray_multiplication():
+                y = y.reshape((*self.dimsd, -1))
+            return y
+        raise ValueError(
+            "Wrong shape.\nFor vector multiplication, expects either a 1d array or, an "
+            "ndarray of size `dims` when `dims` and `dimsd` both are available.\n"
+            "For matrix multiplication, expects a 2d array with its first dimension is "
+            f"equal to {self.shape[1]}.\nInstead, received an array of shape {xshape}.")
+
+    def __mul__(self, x):
+        return self.dot(x)
+
+    def __matmul__(self, other):
+        if _isscalar(other):
+            raise ValueError("Scalar not allowed, use * instead")
+        return self.__mul__(other)
+
+    def __rmul__(self, x):
+        if _isscalar(x):
+            return _ScaledLinearOperator(self, x)
+        return NotImplemented
+
+    def __neg__(self):
+        return _ScaledLinearOperator(self, -1)
+
+    def __add__(self, x):
+        if isinstance(x, LinearOperator):
+            return _SumLinearOperator(self, x)
+        return NotImplemented
+
+    def __sub__(self, x):
+        return self.__add__(-x)
+
+    # ---- adjoint / transpose (LO:462-481) ----
+    def _adjoint(self):
+        return _AdjointLinearOperator(self)
+
+    def adjoint(self):
+        return self._adjoint()
+
+    H = property(adjoint)
+
+    def transpose(self):
+        return _TransposedLinearOperator(self)
+
+    T = property(transpose)
+
+    def todense(self):
+        """Dense matrix by applying the operator to the identity columns."""
+        N = self.shape[1]
+        return np.stack([np.asarray(self.matvec(np.eye(N, dtype=self.dtype)[i]))
+                         for i in range(N)], axis=1)
+
+
+class _AdjointLinearOperator(LinearOperator):
+    """LO:1355-1375."""
+
+    def __init__(self, A):
+        super().__init__(dtype=A.dtype, shape=(A.shape[1], A.shape[0]), dims=A.dimsd,
+                         dimsd=A.dims, clinear=A.clinear, forceflat=A.forceflat,
+                         name=A.name)
+        self.A = A
+        self.args = (A,)
+
+    def _matvec(self, x):
+        return self.A._rmatvec(x)
+
+    def _rmatvec(self, x):
+        return self.A._matvec(x)
+
+    def _matmat(self, X):
+        return self.A._rmatmat(X)
+
+    def _rmatmat(self, X):
+        return self.A._matmat(X)
+
+
+class _TransposedLinearOperator(_AdjointLinearOperator):
+    """Real operators only: the transpose equals the adjoint."""
+
+    def __init__(self, A):
+        if np.issubdtype(A.dtype, np.complexfloating):
+            raise NotImplementedError("complex operators are not supported")
+        super().__init__(A)
+
+
+class _ScaledLinearOperator(LinearOperator):
+    def __init__(self, A, alpha):
+        super().__init__(dtype=A.dtype, shape=A.shape, dims=A.dims, dimsd=A.dimsd,
+                         clinear=A.clinear, forceflat=A.forceflat)
+        self.A, self.alpha = A, alpha
+        self.args = (A, alpha)
+
+    def _matvec(self, x):
+        return self.alpha * self.A._matvec(x)
+
+    def _rmatvec(self, x):
+        return self.alpha * self.A._rmatvec(x)
+
+
+class _SumLinearOperator(LinearOperator):
+    def __init__(self, A, B):
+        if A.shape != B.shape:
+            raise ValueError(f"cannot add {A} and {B}: shape mismatch")
+        super().__init__(dtype=np.promote_types(A.dtype, B.dtype), shape=A.shape,
+                         dims=A.dims, dimsd=A.dimsd, clinear=A.clinear and B.clinear,
+                         forceflat=A.forceflat)
+        self.A, self.B = A, B
+        self.args = (A, B)
+
+    def _matvec(self, x):
+        return self.A._matvec(x) + self.B._matvec(x)
+
+    def _rmatvec(self, x):
+        return self.A._rmatvec(x) + self.B._rmatvec(x)
+
+
+class _ProductLinearOperator(LinearOperator):
+    def __init__(self, A, B):
+        if A.shape[1] != B.shape[0]:
+            raise ValueError(f"cannot multiply {A} and {B}: shape mismatch")
+        super().__init__(dtype=np.promote_types(A.dtype, B.dtype),
+                         shape=(A.shape[0], B.shape[1]), dims=B.dims, dimsd=A.dimsd,
+                         clinear=A.clinear and B.clinear, forceflat=A.forceflat)
+        self.A, self.B = A, B
+        self.args = (A, B)
+
+    def _matvec(self, x):
+        return self.A._matvec(self.B._matvec(x))
+
+    def _rmatvec(self, x):
+        return self.B._rmatvec(self.A._rmatvec(x))
+
+
+def aslinearoperator(Op):
+    if isinstance(Op, LinearOperator):
+        return Op
+    return LinearOperator(Op)

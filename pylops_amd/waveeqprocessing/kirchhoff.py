@@ -1,0 +1,409 @@
+"""Kirchhoff demigration / migration operator on MI355X.
+
+Drop-in for ``pylops.waveeqprocessing.Kirchhoff``
+(pylops/waveeqprocessing/kirchhoff.py, K:234-1143): same constructor
+signature, attributes, ``dims``/``dimsd`` and exceptions.  The engine
+kernels (K:683-1001) and the wavelet Convolve1D (K:407-409) run as HIP
+kernels of libkirchx.so on the device; tables are built in float64 on the
+device and stored as per-tile integer sample bases plus fp32 / fp16 / fp64
+sample offsets.
+"""
+import ctypes
+import logging
+import warnings
+
+import numpy as np
+import torch
+
+from pylops_amd import _lib
+from pylops_amd.linearoperator import LinearOperator
+from pylops_amd.signalprocessing.convolve1d import Convolve1D
+from pylops_amd.utils._internal import _value_or_sized_to_array
+from pylops_amd.utils.backend import is_torch, like_input, to_device
+from pylops_amd.utils.decorators import reshaped
+from pylops_amd.utils.tapers import taper
+from pylops_amd.waveeqprocessing import _tables as T
+
+__all__ = ["Kirchhoff"]
+
+logger = logging.getLogger(__name__)
+
+_ENGINES = ("hip", "cuda", "numba", "numpy")
+# small attributes whose reassignment invalidates the device tables
+_PLAIN_ATTRS = ("vel", "aperture", "angleaperture", "six", "rix", "tile", "rblock",
+                "tabledtype")
+
+
+class _Table:
+    """A table attribute: stored as given (numpy or torch), read back as a
+    numpy array like the reference's attributes, and any reassignment marks
+    the device copy stale (tests reassign e.g. ``Op.trav_srcs = ...``)."""
+
+    def __set_name__(self, owner, name):
+        self.name = name
+
+    def __get__(self, obj, objtype=None):
+        if obj is None:
+            return self
+        if self.name not in obj._tab:
+            raise AttributeError(self.name)
+        host = obj._tab_host.get(self.name)
+        if host is None:
+            host = _host(obj._tab[self.name])
+            obj._tab_host[self.name] = host
+        return host
+
+    def __set__(self, obj, value):
+        obj._tab[self.name] = value
+        obj._tab_host.pop(self.name, None)
+        object.__setattr__(obj, "_dirty", True)
+
+
+class Kirchhoff(LinearOperator):
+    r"""Kirchhoff demigration operator (see the reference docstring,
+    K:40-232).
+
+    Parameters are those of the reference: ``z, x, t, srcs, recs, vel, wav,
+    wavcenter, y=None, mode="eikonal", wavfilter=False, dynamic=False,
+    trav=None, amp=None, aperture=None, angleaperture=90.0, snell=None,
+    engine="hip", dtype="float64", name="K"``.  ``engine`` accepts the
+    reference names ("numpy", "numba", "cuda"); every engine runs the HIP
+    kernels.  Extra keyword-only parameters:
+
+    tabledtype : ``"float16"`` stores traveltime offsets in half precision
+        (float32 operators only); default is the operator dtype.
+    tile : ``(tz, tx, ty)`` image tile of 64 points.
+    device : torch device of the tables (default: current HIP device).
+    rblock : receivers per forward workgroup (0 = automatic).
+    """
+
+    trav_srcs = _Table()
+    trav_recs = _Table()
+    trav = _Table()
+    amp_srcs = _Table()
+    amp_recs = _Table()
+    angle_srcs = _Table()
+    angle_recs = _Table()
+
+    def __init__(self, z, x, t, srcs, recs, vel, wav, wavcenter, y=None,
+                 mode="eikonal", wavfilter=False, dynamic=False, trav=None, amp=None,
+                 aperture=None, angleaperture=90.0, snell=None, engine="hip",
+                 dtype="float64", name="K", *, tabledtype=None, tile=None,
+                 device=None, rblock=0):
+        object.__setattr__(self, "_dirty", True)
+        object.__setattr__(self, "_tab", {})
+        object.__setattr__(self, "_tab_host", {})
+        warnings.warn(
+            "Kirchhoff tables are best passed as (sources, receivers) tuples; "
+            "the single combined table is kept for back-compatibility only.",
+            FutureWarning)
+        if engine not in _ENGINES:
+            raise KeyError("engine must be hip, numpy, numba or cuda")
+        _lib.require_device()
+        self.device = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        dev = self.device
+
+        (self.ndims, _, dims, self.ny, self.nx, self.nz, ns, nr, _, _, _, dsamp,
+         _) = Kirchhoff._identify_geometry(z, x, srcs, recs, y=y)
+        self.dt = t[1] - t[0]
+        self.nt = len(t)
+        self.dynamic = dynamic
+        if self.dynamic:
+            self.six, self.rix = T.source_receiver_indices(self.ndims, x, srcs, recs)
+
+        self.travsrcrec = True
+        ni = int(np.prod(dims))
+        if mode in ("analytic", "eikonal"):
+            if mode == "eikonal":
+                raise NotImplementedError(
+                    "mode='eikonal' needs an eikonal solver, which is not provided")
+            if not isinstance(vel, (float, int)):
+                raise ValueError("vel must be scalar for mode=analytical")
+            ts, tr, ds, dr, gs, gr = T.analytic(z, x, srcs, recs, vel, y, dims, dsamp, dev)
+            self._set("trav_srcs", ts)
+            self._set("trav_recs", tr)
+            if self.dynamic:
+                a_s, a_r, self.maxdist = T.amplitudes(self.ndims, ds, dr)
+                self._set("amp_srcs", a_s)
+                self._set("amp_recs", a_r)
+            del ds, dr
+        elif mode == "byot":
+            if isinstance(trav, tuple):
+                self._set("trav_srcs", trav[0])
+                self._set("trav_recs", trav[1])
+            else:
+                self.travsrcrec = False
+                self._set("trav", trav)
+            if self.dynamic and not self.travsrcrec:
+                raise NotImplementedError("separate traveltime tables must be provided "
+                                          "when selecting mode=dynamic")
+            if self.dynamic:
+                if isinstance(amp, tuple):
+                    self._set("amp_srcs", amp[0])
+                    self._set("amp_recs", amp[1])
+                else:
+                    raise NotImplementedError("separate amplitude tables must be provided ")
+                gs = T.gradient(T.as_f64(trav[0], dev), dims, None)
+                gr = T.gradient(T.as_f64(trav[1], dev), dims, None)
+        else:
+            raise NotImplementedError("method must be analytic, eikonal or byot")
+
+        if self.dynamic:
+            angs, angr = T.angles(self.ndims, gs, gr, ni)
+            self._set("angle_srcs", angs)
+            self._set("angle_recs", angr)
+            del gs, gr
+
+        if wavfilter:
+            self.wav = self._wavelet_reshaping(np.asarray(wav), self.dt, srcs.shape[0],
+                                               recs.shape[0], self.ndims)
+        else:
+            self.wav = wav
+        self.cop = Convolve1D((ns * nr, self.nt), h=np.asarray(self.wav), offset=wavcenter,
+                              axis=1, dtype=dtype)
+        self.aperturetap = taper(41, 20, "hanning")[20:]
+
+        if aperture is not None and self.ndims == 3:
+            aperture = None
+            warnings.warn("Aperture is forced to None as currently not implemented in 3D")
+        if aperture is not None:
+            warnings.warn("Aperture is currently defined as ratio of offset over depth, "
+                          "and may be not ideal for highly heterogeneous media")
+        ap = (self.nx + 1,) if aperture is None else _value_or_sized_to_array(aperture)
+        if len(ap) == 1:
+            ap = np.array([0.8 * ap[0], ap[0]])
+        self.aperture = ap
+        angleaperture = [0.0, 1000.0] if angleaperture is None else angleaperture
+        angap = np.deg2rad(_value_or_sized_to_array(angleaperture))
+        if len(angap) == 1:
+            angap = np.array([0.8 * angap[0], angap[0]])
+        self.angleaperture = angap
+
+        self.ns, self.nr = ns, nr
+        self.nsnr = ns * nr
+        self.ni = ni
+        dims = tuple(dims) if self.ndims == 2 else (dims[0] * dims[1], dims[2])
+        super().__init__(dtype=np.dtype(dtype), dims=dims, dimsd=(ns, nr, self.nt), name=name)
+        if self.dynamic:
+            self.vel = (vel.flatten() if not isinstance(vel, (float, int))
+                        else vel * np.ones(ni))
+
+        if self.dtype not in (np.float32, np.float64):
+            raise NotImplementedError("dtype must be float32 or float64")
+        if tabledtype is None:
+            self.tabledtype = self.dtype
+        else:
+            self.tabledtype = np.dtype(tabledtype)
+            if self.tabledtype == np.float16 and self.dtype != np.float32:
+                raise NotImplementedError("float16 tables need a float32 operator")
+            if self.tabledtype not in (np.float16, self.dtype):
+                raise NotImplementedError(f"tabledtype {self.tabledtype} with dtype {self.dtype}")
+        self.tile = tuple(tile) if tile is not None else T.default_tile(self.ndims, self.nz)
+        self.rblock = int(rblock)
+        self.engine = engine
+        if engine == "cuda" and not self.travsrcrec:
+            raise NotImplementedError("engine='cuda' not implemented for traveltimes "
+                                      "provided in one table")
+        if engine in ("numpy", "numba"):
+            logger.info("engine=%s runs the HIP kernels of pylops_amd", engine)
+        self._build_device_tables()
+
+    # ------------------------------------------------------------------
+    # table attributes: reassignment marks the device copy stale
+    # ------------------------------------------------------------------
+    def _set(self, name, value):
+        setattr(self, name, value)
+
+    def __setattr__(self, name, value):
+        object.__setattr__(self, name, value)
+        if name in _PLAIN_ATTRS:
+            object.__setattr__(self, "_dirty", True)
+
+    @property
+    def itrav(self):
+        """Integer sample of the combined table (K:395-398), on request."""
+        if self.travsrcrec:
+            raise AttributeError("itrav exists only for a single combined table")
+        return (np.asarray(_host(self.trav)) / self.dt).astype("int32")
+
+    @property
+    def travd(self):
+        if self.travsrcrec:
+            raise AttributeError("travd exists only for a single combined table")
+        q = np.asarray(_host(self.trav)) / self.dt
+        return q - q.astype("int32")
+
+    # ------------------------------------------------------------------
+    # reference static helpers
+    # ------------------------------------------------------------------
+    @staticmethod
+    def _identify_geometry(z, x, srcs, recs, y=None):
+        """K:457-500."""
+        ns, nr = srcs.shape[1], recs.shape[1]
+        nz, nx = len(z), len(x)
+        dz = np.abs(z[1] - z[0])
+        dx = np.abs(x[1] - x[0])
+        if y is None:
+            return (2, 0, np.array([nx, nz]), 1, nx, nz, ns, nr, None, dx, dz,
+                    np.array([dx, dz]), np.array([x[0], z[0]]))
+        ny = len(y)
+        dy = np.abs(y[1] - y[0])
+        return (3, 1, np.array([ny, nx, nz]), ny, nx, nz, ns, nr, dy, dx, dz,
+                np.array([dy, dx, dz]), np.array([y[0], x[0], z[0]]))
+
+    @staticmethod
+    def _traveltime_table(z, x, srcs, recs, vel, y=None, mode="eikonal"):
+        """K:502-657 for mode="analytic" (numpy outputs, float64).  The
+        tables are computed with torch on the current HIP device when one
+        is visible and on the host otherwise."""
+        (ndims, _, dims, _, _, _, _, _, _, _, _, dsamp, _) = \
+            Kirchhoff._identify_geometry(z, x, srcs, recs, y=y)
+        if mode == "analytic":
+            if not isinstance(vel, (float, int)):
+                raise ValueError("vel must be scalar for mode=analytical")
+        elif mode == "eikonal":
+            raise NotImplementedError(
+                "mode='eikonal' needs an eikonal solver, which is not provided")
+        else:
+            raise NotImplementedError("method must be analytic or eikonal")
+        dev = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
+        out = T.analytic(z, x, srcs, recs, vel, y, dims, dsamp, dev)
+        ts, tr, ds, dr = (o.cpu().numpy() for o in out[:4])
+        gs = [g.cpu().numpy() for g in out[4]]
+        gr = [g.cpu().numpy() for g in out[5]]
+        return ts, tr, ds, dr, gs, gr
+
+    def _wavelet_reshaping(self, wav, dt, dimsrc, dimrec, dimv):
+        """K:659-681: sqrt(i w) shaping in 2D, -i w in 3D."""
+        f = np.fft.rfftfreq(len(wav), dt)
+        W = np.fft.rfft(wav, n=len(wav))
+        if dimsrc == 2 and dimv == 2:
+            Wfilt = W * np.sqrt(1j * 2 * np.pi * f)
+        elif (dimsrc == 2 or dimrec == 2) and dimv == 3:
+            raise NotImplementedError("2.D wavelet currently not available")
+        elif dimsrc == 3 and dimrec == 3 and dimv == 3:
+            Wfilt = W * (-1j * 2 * np.pi * f)
+        return np.fft.irfft(Wfilt, n=len(wav))
+
+    # ------------------------------------------------------------------
+    # device tables and plan
+    # ------------------------------------------------------------------
+    def _fmt(self):
+        if self.tabledtype == np.float16:
+            return _lib.KX_F16
+        return _lib.KX_F64 if self.dtype == np.float64 else _lib.KX_F32
+
+    def _build_device_tables(self):
+        dev = self.device
+        ny = self.ny if self.ndims == 3 else 1
+        plan = T.make_geom_plan(self.nx, ny, self.nz, self.tile)
+        fmt = self._fmt()
+        inv_dt = 1.0 / float(self.dt)
+        keep = {}
+        if self.travsrcrec:
+            ts = T.as_f64(self._tab["trav_srcs"], dev)
+            tr = T.as_f64(self._tab["trav_recs"], dev)
+            ns, nr = self.ns, self.nr
+        else:
+            ts = T.as_f64(self._tab["trav"], dev)
+            tr = torch.zeros((self.ni, 1), dtype=torch.float64, device=dev)
+            ns, nr = self.nsnr, 1
+        if ts.shape != (self.ni, ns) or tr.shape != (self.ni, nr):
+            raise ValueError(f"traveltime tables of shape {tuple(ts.shape)} / "
+                             f"{tuple(tr.shape)} do not match (ni, n) = ({self.ni}, {ns}) / "
+                             f"({self.ni}, {nr})")
+        neg = bool((ts.min() + tr.min()) < 0)
+        keep["off_s"], keep["base_s"] = T.encode(plan, ts, inv_dt, fmt)
+        keep["off_r"], keep["base_r"] = T.encode(plan, tr, inv_dt, fmt)
+        del ts, tr
+        wfmt = _lib.KX_F64 if self.dtype == np.float64 else _lib.KX_F32
+        if self.dynamic:
+            vel = T.as_f64(self.vel, dev).reshape(-1)
+            amin, amax = float(self.angleaperture[0]), float(self.angleaperture[1])
+            tab = self._tab
+            ps, qs = T.side_weights(T.as_f64(tab["amp_srcs"], dev),
+                                    T.as_f64(tab["angle_srcs"], dev),
+                                    amin, amax, self.aperturetap, scale=2.0 / vel)
+            pr, qr = T.side_weights(T.as_f64(tab["amp_recs"], dev),
+                                    T.as_f64(tab["angle_recs"], dev),
+                                    amin, amax, self.aperturetap)
+            keep["wp_s"] = T.tile(plan, ps, wfmt)
+            keep["wq_s"] = T.tile(plan, qs, wfmt)
+            keep["wp_r"] = T.tile(plan, pr, wfmt)
+            keep["wq_r"] = T.tile(plan, qr, wfmt)
+            izmin = T.aperture_izmin(self.six, self.rix, self.nz, float(self.aperture[1]))
+            keep["izmin"] = torch.as_tensor(izmin, device=dev)
+        tdt = torch.float64 if self.dtype == np.float64 else torch.float32
+        keep["h"] = torch.as_tensor(np.asarray(self.cop.h, dtype=np.float64), dtype=tdt,
+                                    device=dev)
+        keep["hstar"] = torch.as_tensor(np.asarray(self.cop.hstar, dtype=np.float64),
+                                        dtype=tdt, device=dev)
+        plan.dtype = wfmt
+        plan.tab_fmt = fmt
+        plan.dynamic = int(self.dynamic)
+        plan.ns, plan.nr, plan.nt = ns, nr, self.nt
+        plan.neg_trav = int(neg)
+        plan.rblock = self.rblock
+        plan.nh = keep["h"].numel()
+        for k, v in keep.items():
+            setattr(plan, k, v.data_ptr())
+        object.__setattr__(self, "_dev_tables", keep)
+        object.__setattr__(self, "_plan", plan)
+        object.__setattr__(self, "_dirty", False)
+
+    @property
+    def plan(self):
+        """The kx_plan (ctypes) the kernels are launched with."""
+        if self._dirty:
+            self._build_device_tables()
+        return self._plan
+
+    def _work(self, device, tdtype):
+        return torch.empty(self.nsnr * self.nt, dtype=tdtype, device=device)
+
+    def _launch(self, fn, x, nout, what):
+        plan = self.plan
+        xd = to_device(x.reshape(-1), self.dtype, self.device)
+        out = torch.empty(nout, dtype=xd.dtype, device=self.device)
+        work = self._work(self.device, xd.dtype)
+        rc = fn(ctypes.byref(plan), _lib.ptr(xd), _lib.ptr(out), _lib.ptr(work),
+                _lib.stream_ptr(self.device))
+        _lib.check(rc, what)
+        return like_input(out, x)
+
+    # ------------------------------------------------------------------
+    # products (K:1048-1143)
+    # ------------------------------------------------------------------
+    @reshaped
+    def _matvec(self, x):
+        return self._launch(_lib.load().kx_forward, x, self.nsnr * self.nt, "kx_forward")
+
+    @reshaped
+    def _rmatvec(self, x):
+        return self._launch(_lib.load().kx_adjoint, x, self.ni, "kx_adjoint")
+
+    # kernels alone, without the wavelet (the engine seam, K:1003-1046)
+    def spread(self, m):
+        """The forward kernel alone (no wavelet): traces (ns*nr, nt)."""
+        plan = self.plan
+        md = to_device(m.reshape(-1), self.dtype, self.device)
+        out = torch.empty(self.nsnr * self.nt, dtype=md.dtype, device=self.device)
+        rc = _lib.load().kx_spread(ctypes.byref(plan), _lib.ptr(md), _lib.ptr(out),
+                                   _lib.stream_ptr(self.device))
+        _lib.check(rc, "kx_spread")
+        return like_input(out, m)
+
+    def gather(self, d):
+        """The adjoint kernel alone (no wavelet correlation): image (ni,)."""
+        plan = self.plan
+        dd = to_device(d.reshape(-1), self.dtype, self.device)
+        out = torch.empty(self.ni, dtype=dd.dtype, device=self.device)
+        rc = _lib.load().kx_gather(ctypes.byref(plan), _lib.ptr(dd), _lib.ptr(out),
+                                   _lib.stream_ptr(self.device))
+        _lib.check(rc, "kx_gather")
+        return like_input(out, d)
+
+
+def _host(a):
+    return a.detach().cpu().numpy() if is_torch(a) else np.asarray(a)

@@ -1,0 +1,190 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+declared symbol, the ctypes plan matches the C struct layout, and the
+table / weight construction reproduces the reference's per-triplet weights
+(checked against the oracle, itself pinned by golden vectors)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests import geometry as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kirchx.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(kx_\w+)\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pylops_amd import _lib
+    L = _lib.load()
+    names = _declared()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.EXPORTS)
+    assert L.kx_version() == _lib.KX_ABI_VERSION
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIBPATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(kx_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_plan_struct_layout_matches_header(tmp_path):
+    """Compile the header with gcc and compare sizeof/offsetof with ctypes."""
+    import ctypes
+    from pylops_amd import _lib
+    fields = [f[0] for f in _lib.KxPlan._fields_]
+    prog = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
+            'int main(void){', 'printf("%zu\\n", sizeof(kx_plan));']
+    prog += [f'printf("%zu\\n", offsetof(kx_plan, {f}));' for f in fields]
+    prog += ['return 0;}']
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(prog))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", str(c), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    assert vals[0] == ctypes.sizeof(_lib.KxPlan)
+    for f, off in zip(fields, vals[1:]):
+        assert getattr(_lib.KxPlan, f).offset == off, f
+
+
+def test_error_mapping():
+    from pylops_amd import _lib
+    import ctypes
+    L = _lib.load()
+    p = _lib.KxPlan()
+    p.abi_version = 99
+    rc = L.kx_spread(ctypes.byref(p), None, None, None)
+    assert rc == _lib.KX_EABI
+    with pytest.raises(ValueError, match="abi_version"):
+        _lib.check(rc, "kx_spread")
+    p.abi_version = _lib.KX_ABI_VERSION
+    p.nx, p.ny, p.nz, p.tz, p.tx, p.ty = 4, 1, 4, 8, 4, 1
+    rc = L.kx_spread(ctypes.byref(p), None, None, None)
+    assert rc == _lib.KX_ESHAPE
+    rc = L.kx_conv(None, None, 1, 10, None, 3, 0, None)
+    assert rc == _lib.KX_EINVAL
+
+
+def test_analytic_tables_match_oracle():
+    from pylops_amd.waveeqprocessing import _tables as T
+    for dim3 in (False, True):
+        s, r = (G.s3d, G.r3d) if dim3 else (G.s2d, G.r2d)
+        y = G.y if dim3 else None
+        geo = O.identify_geometry(G.z, G.x, s, r, y)
+        ts, tr, ds, dr, gs, gr = T.analytic(G.z, G.x, s, r, G.v0, y, geo["dims"],
+                                            geo["dsamp"], torch.device("cpu"))
+        ots, otr, ods, odr, ogs, ogr = O.analytic_tables(G.z, G.x, s, r, G.v0, y)
+        # torch's CPU sqrt is not always correctly rounded (1 ulp); the device
+        # path uses IEEE sqrt and division
+        np.testing.assert_allclose(ts.numpy(), ots, rtol=5e-16, atol=0)
+        np.testing.assert_allclose(tr.numpy(), otr, rtol=5e-16, atol=0)
+        for a, b in zip(gs + gr, ogs + ogr):
+            np.testing.assert_allclose(a.numpy(), b, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("name", ["dyn2d_f64", "dyn2d_ap_f64", "dyn2d_ap_f32",
+                                  "dyn2d_byot_f64", "dyn3d_f64"])
+def test_dynamic_weight_factorisation(name):
+    """Ps*Pr + Qs*Qr gated by the depth aperture equals the reference's
+    per-triplet weight 2 cos((ts-tr)/2) As Ar / v * aptap (K:820-889) for
+    every (image point, source, receiver) of the golden geometries."""
+    from pylops_amd.waveeqprocessing import _tables as T
+    g = np.load(os.path.join(G.GOLDEN, name + ".npz"))
+    kw = G.golden_kwargs(name, g)
+    orc = O.KirchhoffOracle(**kw)
+    cpu = torch.device("cpu")
+    amin, amax = orc.angleaperture
+    vel = torch.as_tensor(orc.vel)
+    ps, qs = T.side_weights(torch.as_tensor(g["amp_srcs"]), torch.as_tensor(g["angle_srcs"]),
+                            amin, amax, orc.aptap, scale=2.0 / vel)
+    pr, qr = T.side_weights(torch.as_tensor(g["amp_recs"]), torch.as_tensor(g["angle_recs"]),
+                            amin, amax, orc.aptap)
+    w = (ps[:, :, None] * pr[:, None, :] + qs[:, :, None] * qr[:, None, :]).numpy()
+    ni, ns, nr = w.shape
+    izmin = T.aperture_izmin(g["six"], g["rix"], orc.nz, float(orc.aperture[1]))
+    iz = np.arange(ni) % orc.nz
+    gate = iz[:, None, None] >= izmin.reshape(ns, nr)[None]
+    w = np.where(gate, w, 0.0)
+    # reference weight, evaluated with the oracle's restatement of K:820-880
+    a_s = np.abs(g["angle_srcs"])[:, :, None]
+    a_r = np.abs(g["angle_recs"])[:, None, :]
+    dang = amax - amin
+    tap = orc.aptap
+
+    def taper_of(a):
+        idx = np.clip(np.floor_divide(20 * (a - amin), dang), 0, 20).astype(int)
+        return np.where(a >= amin, tap[idx], 1.0)
+
+    ok_ang = (a_s < amax) & (a_r < amax)
+    ap = np.abs(g["six"] - g["rix"]).reshape(ns, nr)[None] / (iz[:, None, None] + 1)
+    ok = ok_ang & (ap < orc.aperture[1])
+    cosang = np.cos((g["angle_srcs"][:, :, None] - g["angle_recs"][:, None, :]) / 2.0)
+    ref = (2.0 * cosang * g["amp_srcs"][:, :, None] * g["amp_recs"][:, None, :]
+           / orc.vel[:, None, None]) * taper_of(a_s) * taper_of(a_r)
+    ref = np.where(ok, ref, 0.0)
+    np.testing.assert_allclose(w, ref, rtol=1e-12, atol=1e-15 * np.abs(ref).max())
+    del cpu
+
+
+def test_aperture_izmin_is_exact_threshold():
+    from pylops_amd.waveeqprocessing import _tables as T
+    six = np.array([0, 3, 7, 11, 5])
+    rix = np.array([0, 0, 2, 0, 11])
+    for apmax in (0.5, 1.0, 2.0, 13.0):
+        izmin = T.aperture_izmin(six, rix, 20, apmax)
+        for k in range(5):
+            for iz in range(20):
+                assert (iz >= izmin[k]) == (abs(six[k] - rix[k]) / (iz + 1) < apmax)
+
+
+def test_kirchhoff_needs_a_device():
+    """No CPU fallback: construction fails loudly without a HIP device."""
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is visible")
+    import pylops_amd
+    with pytest.raises(RuntimeError, match="HIP device"):
+        pylops_amd.Kirchhoff(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                             mode="analytic")
+
+
+def test_linearoperator_semantics_numpy():
+    """pytests/test_linearoperator.py:331-388 restated on a dense operator."""
+    from pylops_amd import LinearOperator
+
+    class Dense(LinearOperator):
+        def __init__(self, A, dims, dimsd):
+            super().__init__(dtype=A.dtype, dims=dims, dimsd=dimsd)
+            self.A = A
+
+        def _matvec(self, x):
+            return self.A @ x
+
+        def _rmatvec(self, y):
+            return self.A.T @ y
+
+    A = np.arange(24.0).reshape(6, 4)
+    Op = Dense(A, (2, 2), (3, 2))
+    x = np.ones((2, 2))
+    y = Op @ x
+    assert y.shape == (3, 2)
+    np.testing.assert_allclose(y.ravel(), A @ np.ones(4))
+    assert (Op.H @ y).shape == (2, 2)
+    assert Op.matvec_count == 1 and Op.rmatvec_count == 0
+    X = np.ones((4, 3))
+    Y = Op.matmat(X)
+    assert Y.shape == (6, 3)
+    assert Op.matmat_count == 1 and Op.matvec_count == 1
+    with pytest.raises(ValueError):
+        Op.matvec(np.ones(5))
+    np.testing.assert_allclose((2 * Op) @ np.ones(4), 2 * A @ np.ones(4))
+    np.testing.assert_allclose((Op.H * Op) @ np.ones(4), A.T @ A @ np.ones(4))

@@ -1,0 +1,206 @@
+"""GPU parity of the HIP Kirchhoff operator against the CPU oracle and the
+reference's golden vectors.
+
+Tolerances (relative L2):
+  float64 operator vs float64 oracle / golden     1e-10
+  float32 operator vs oracle                      1e-5
+  float16-offset tables vs float64 oracle         1e-3
+  dottest                                         1e-6 (float64), 1e-5 (float32)
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _K(**kw):
+    import pylops_amd
+    return pylops_amd.Kirchhoff(**kw)
+
+
+def _load(name):
+    return np.load(os.path.join(G.GOLDEN, name + ".npz"))
+
+
+def _dottest(Op, rtol, seed=0):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal(Op.shape[1]).astype(Op.dtype)
+    v = rng.standard_normal(Op.shape[0]).astype(Op.dtype)
+    yy = np.dot(np.asarray(Op.matvec(u), dtype=np.float64), v.astype(np.float64))
+    xx = np.dot(u.astype(np.float64), np.asarray(Op.rmatvec(v), dtype=np.float64))
+    assert abs(xx - yy) <= rtol * max(abs(xx), abs(yy)), (xx, yy)
+
+
+@pytest.mark.parametrize("name", sorted(G.golden_cases()))
+def test_golden(name):
+    g = _load(name)
+    kw = G.golden_kwargs(name, g)
+    Op = _K(**kw)
+    oracle = O.KirchhoffOracle(**kw)
+    f32 = Op.dtype == np.float32
+    tol = 1e-5 if f32 else 1e-10
+    fwd = Op @ g["m"]
+    adj = Op.H @ g["d"]
+    assert G.relerr(fwd, g["fwd"]) < tol
+    assert G.relerr(adj, g["adj"]) < tol
+    if "spread" in g.files:
+        assert G.relerr(Op.spread(g["m"]), g["spread"]) < tol
+        assert G.relerr(Op.gather(g["dk"]), oracle.gather(g["dk"])) < tol
+    _dottest(Op, 1e-5 if f32 else 1e-9)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("dim3", [False, True])
+def test_vs_oracle_par(dtype, dynamic, dim3):
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s3d if dim3 else G.s2d,
+              recs=G.r3d if dim3 else G.r2d, vel=G.v0, wav=G.wav, wavcenter=G.wavc,
+              mode="analytic", dynamic=dynamic, dtype=dtype, angleaperture=60)
+    if dim3:
+        kw["y"] = G.y
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    rng = np.random.default_rng(5)
+    m = rng.standard_normal(Op.shape[1])
+    d = rng.standard_normal(Op.shape[0])
+    tol = 1e-5 if dtype == "float32" else 1e-10
+    assert G.relerr(Op.matvec(m.astype(dtype)), orc.matvec(m)) < tol
+    assert G.relerr(Op.rmatvec(d.astype(dtype)), orc.rmatvec(d)) < tol
+    _dottest(Op, 1e-5 if dtype == "float32" else 1e-9)
+
+
+def test_reference_dottests():
+    """pytests/test_kirchhoff.py:215-317 (analytic and byot, 2D and 3D)."""
+    from pylops_amd.utils import dottest
+    for dim3 in (False, True):
+        s, r = (G.s3d_par, G.r3d_par) if dim3 else (G.s2d, G.r2d)
+        yy = G.y if dim3 else None
+        Op = _K(z=G.z, x=G.x, t=G.t, srcs=s, recs=r, vel=G.v0, wav=G.wav,
+                wavcenter=G.wavc, y=yy, mode="analytic")
+        assert dottest(Op, Op.shape[0], Op.shape[1])
+        trav = (Op.trav_srcs[:, :, None] + Op.trav_recs[:, None, :]).reshape(Op.ni, -1)
+        Op1 = _K(z=G.z, x=G.x, t=G.t, srcs=s, recs=r, vel=G.v0, wav=G.wav,
+                 wavcenter=G.wavc, y=yy, mode="byot", trav=trav)
+        assert dottest(Op1, Op1.shape[0], Op1.shape[1])
+        rng = np.random.default_rng(1)
+        m = rng.standard_normal(Op.shape[1])
+        d = rng.standard_normal(Op.shape[0])
+        np.testing.assert_allclose(Op @ m, Op1 @ m, atol=1e-10)
+        np.testing.assert_allclose(Op.H @ d, Op1.H @ d, atol=1e-10)
+
+
+def test_attribute_reassignment_rebuilds_tables():
+    Op = _K(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+            wavcenter=G.wavc, mode="analytic")
+    m = np.random.default_rng(2).standard_normal(Op.shape[1])
+    d0 = Op @ m
+    Op.trav_srcs = np.asarray(Op.trav_srcs) * 1.05
+    orc = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc, mode="byot",
+                            trav=(np.asarray(Op.trav_srcs), np.asarray(Op.trav_recs)))
+    d1 = Op @ m
+    assert G.relerr(d1, orc.matvec(m)) < 1e-10
+    assert G.relerr(d0, d1) > 1e-3
+
+
+def _config1(dtype="float32", tabledtype=None):
+    """2D constant-velocity LSM workload: grid 101 x 100 at 10 m, nt=651 at
+    4 ms, 11 sources and 51 receivers on the surface, 20 Hz Ricker."""
+    x = np.arange(101) * 10.0
+    z = np.arange(100) * 10.0
+    t = np.arange(651) * 0.004
+    srcs = np.vstack((np.linspace(x[0], x[-1], 11), np.zeros(11)))
+    recs = np.vstack((np.linspace(x[0], x[-1], 51), np.zeros(51)))
+    wav, _, wc = O.ricker(t[:41], f0=20)
+    return dict(z=z, x=x, t=t, srcs=srcs, recs=recs, vel=2000.0,
+                wav=wav.astype(np.float32), wavcenter=wc, mode="analytic",
+                dtype=dtype), tabledtype
+
+
+def test_config1_parity_fp32_fp16():
+    kw, _ = _config1()
+    rng = np.random.default_rng(1001)
+    refl = 0.01 * rng.standard_normal((101, 100))
+    for iz in rng.integers(10, 90, 3):
+        refl[:, iz] += rng.uniform(-1, 1)
+    orc64 = O.KirchhoffOracle(**{**kw, "dtype": "float64"})
+    d64 = orc64.matvec(refl.ravel())
+    a64 = orc64.rmatvec(d64)
+    Op = _K(**kw)
+    d32 = Op @ refl.astype(np.float32)
+    a32 = Op.H @ d32
+    assert G.relerr(d32, d64) < 1e-5
+    assert G.relerr(a32, a64) < 1e-5
+    orc32 = O.KirchhoffOracle(**kw)
+    assert G.relerr(d32, orc32.matvec(refl.astype(np.float32))) < 1e-5
+    Op16 = _K(**kw, tabledtype="float16")
+    d16 = Op16 @ refl.astype(np.float32)
+    a16 = Op16.H @ d64.astype(np.float32)
+    assert G.relerr(d16, d64) < 1e-3
+    assert G.relerr(a16, a64) < 1e-3
+    _dottest(Op, 1e-5)
+    _dottest(Op16, 1e-5)
+
+
+def test_negative_traveltimes_truncate_like_reference():
+    """byot tables with negative times take the truncation path (K:743-751):
+    a time in (-1, 0) samples lands on sample 0 with a negative weight.  The
+    shifts are chosen off the sample grid: a time within one float64 ulp of
+    -1 sample is a discontinuity of the truncation rule and rounds either
+    way."""
+    Op0 = _K(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+             wavcenter=G.wavc, mode="analytic")
+    ts = np.asarray(Op0.trav_srcs) - 0.01113
+    tr = np.asarray(Op0.trav_recs) - 0.00871
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="byot", trav=(ts, tr))
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    rng = np.random.default_rng(4)
+    m = rng.standard_normal(Op.shape[1])
+    d = rng.standard_normal(Op.shape[0])
+    assert G.relerr(Op.spread(m), orc.spread(m)) < 1e-10
+    assert G.relerr(Op.gather(d), orc.gather(d)) < 1e-10
+
+
+def test_conv_matches_oracle():
+    import torch
+    from pylops_amd.signalprocessing import Convolve1D
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((7, 300))
+    for nh, off in ((81, 40), (21, 3), (20, 12)):
+        h = rng.standard_normal(nh)
+        C = Convolve1D((7, 300), h=h, offset=off, axis=1, dtype="float64")
+        hp = O.padded_filter(h, off)
+        np.testing.assert_allclose(C @ x, O.conv_same(x, hp), atol=1e-12)
+        np.testing.assert_allclose(C.H @ x, O.conv_same(x, hp[::-1]), atol=1e-12)
+        xt = torch.as_tensor(x, dtype=torch.float32, device="cuda")
+        C32 = Convolve1D((7, 300), h=h, offset=off, axis=1, dtype="float32")
+        y = C32 @ xt
+        assert isinstance(y, torch.Tensor) and y.is_cuda
+        assert G.relerr(y.cpu().numpy(), O.conv_same(x, hp)) < 1e-6
+
+
+def test_dot_fp64_accumulation():
+    import ctypes
+    import torch
+    from pylops_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(7)
+    for n in (1, 1000, 1_000_003):
+        a = rng.standard_normal(n).astype(np.float32)
+        b = rng.standard_normal(n).astype(np.float32)
+        ta = torch.as_tensor(a, device="cuda")
+        tb = torch.as_tensor(b, device="cuda")
+        out = torch.empty(1, dtype=torch.float64, device="cuda")
+        work = torch.empty(L.kx_reduce_workspace_bytes() // 8, dtype=torch.float64, device="cuda")
+        rc = L.kx_dot(_lib.ptr(ta), _lib.ptr(tb), n, _lib.KX_F32, _lib.ptr(out), _lib.ptr(work),
+                      _lib.stream_ptr())
+        _lib.check(rc)
+        ref = np.dot(a.astype(np.float64), b.astype(np.float64))
+        assert abs(float(out) - ref) <= 1e-12 * max(1.0, np.abs(a).dot(np.abs(b)))
+        del ctypes
