@@ -1,0 +1,356 @@
+"""Kirchhoff forward + adjoint throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2d|3d|2dsmall]
+                    [--tables f32|f16] [--no-cpu] [--extra]
+
+A step is one forward (spread + wavelet convolution) of a dense image and
+one adjoint (wavelet correlation + gather) of the observed data, i.e. the
+operator pair one CGLS/LSQR iteration applies.  The dense image is the
+migrated image K^T d (what the solver's forward sees after its first
+adjoint), so no tile of the forward is skipped as empty.
+
+Workloads (synthetic inputs, fixed seeds):
+  2d       2D, analytic constant velocity 2500 m/s, dynamic=True; grid
+           nz=300 x nx=601 at 5 m; nt=1501 at 2 ms; 61 sources and 301
+           receivers on the surface; 25 Hz Ricker; 8 dipping reflectors and
+           point diffractors.                                   (default)
+  3d       3D, byot straight-ray traveltimes for v(z) = 1800 + 0.6 z m/s;
+           grid 120 x 128 x 128 at 10 m; nt=1001 at 4 ms; 5x5 sources and
+           20x20 receivers on the surface; kinematic.
+  2dsmall  2D, 2000 m/s, kinematic; 100 x 101 at 10 m; nt=651 at 4 ms;
+           11 sources, 51 receivers.
+
+Output: one JSON line.  value = 2 * ni * ns * nr / (t_fwd + t_adj) / 1e9
+(Gsamples/s, one sample = one image point x source x receiver triplet).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Kirchhoff fwd+adj Gsamples/s, CGLS s/solve; FP16-table vs FP32 at ≤1e-3 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def ricker(t, f0):
+    t = np.asarray(t)
+    if len(t) % 2 == 0:
+        t = t[:-1]
+    tt = np.concatenate((np.flipud(-t[1:]), t))
+    a = (np.pi * f0 * tt) ** 2
+    w = (1 - 2 * a) * np.exp(-a)
+    return w, int(np.argmax(np.abs(w)))
+
+
+def workload_2d(rng):
+    nx, nz, nt = 601, 300, 1501
+    dx = dz = 5.0
+    x = np.arange(nx) * dx
+    z = np.arange(nz) * dz
+    t = np.arange(nt) * 0.002
+    srcs = np.vstack((np.linspace(x[0], x[-1], 61), np.zeros(61)))
+    recs = np.vstack((np.linspace(x[0], x[-1], 301), np.zeros(301)))
+    wav, wc = ricker(t[:41], 25.0)
+    refl = np.zeros((nx, nz))
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    for _ in range(8):
+        dip = np.deg2rad(rng.uniform(-20, 20))
+        z0 = rng.uniform(0.1, 0.9) * z[-1]
+        zr = z0 + np.tan(dip) * (X - x[nx // 2])
+        refl[np.abs(Z - zr) < dz / 2] += rng.uniform(-1, 1)
+    for _ in range(20):
+        refl[rng.integers(0, nx), rng.integers(nz // 5, nz)] += rng.uniform(-1, 1)
+    kw = dict(z=z, x=x, t=t, srcs=srcs, recs=recs, vel=2500.0, wav=wav.astype(np.float32),
+              wavcenter=wc, mode="analytic", dynamic=True)
+    desc = dict(workload="2d-dynamic-601x300-61x301-nt1501", nx=nx, nz=nz, ns=61, nr=301,
+                nt=nt, dynamic=True)
+    return kw, refl, desc
+
+
+def straight_ray_vz(pts_xyz, src_xyz, v0=1800.0, k=0.6):
+    """Straight-ray traveltime through v(z) = v0 + k z: t = L ln(v2/v1)/(k dz),
+    or L / v when dz = 0."""
+    d = pts_xyz - src_xyz
+    L = np.sqrt((d ** 2).sum(-1))
+    z1 = src_xyz[..., 2]
+    z2 = pts_xyz[..., 2]
+    v1 = v0 + k * z1
+    v2 = v0 + k * z2
+    dzz = z2 - z1
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tt = np.where(np.abs(dzz) > 1e-9, L * np.log(v2 / v1) / (k * dzz), L / v1)
+    return tt
+
+
+def workload_3d(rng, device):
+    import torch
+    ny = nx = 128
+    nz = 120
+    d = 10.0
+    y = np.arange(ny) * d
+    x = np.arange(nx) * d
+    z = np.arange(nz) * d
+    t = np.arange(1001) * 0.004
+    sy, sx = np.meshgrid(np.linspace(y[0], y[-1], 5), np.linspace(x[0], x[-1], 5), indexing="ij")
+    ry, rx = np.meshgrid(np.linspace(y[0], y[-1], 20), np.linspace(x[0], x[-1], 20),
+                         indexing="ij")
+    srcs = np.vstack((sy.ravel(), sx.ravel(), np.zeros(25)))
+    recs = np.vstack((ry.ravel(), rx.ravel(), np.zeros(400)))
+    wav, wc = ricker(t[:41], 20.0)
+    # tables in float64 on the device: (ni, n), reference order (y, x, z)
+    Y, X, Z = torch.meshgrid(*(torch.as_tensor(a, dtype=torch.float64, device=device)
+                               for a in (y, x, z)), indexing="ij")
+    P = torch.stack((Y.reshape(-1), X.reshape(-1), Z.reshape(-1)), -1)
+
+    def table(pts):
+        S = torch.as_tensor(pts.T, dtype=torch.float64, device=device)  # (n, 3) as (y, x, z)
+        diff = P[:, None, :] - S[None, :, :]
+        L = torch.sqrt((diff ** 2).sum(-1))
+        v1 = 1800.0 + 0.6 * S[None, :, 2]
+        v2 = 1800.0 + 0.6 * P[:, None, 2]
+        dzz = P[:, None, 2] - S[None, :, 2]
+        safe = torch.where(dzz.abs() > 1e-9, dzz, torch.ones_like(dzz))
+        return torch.where(dzz.abs() > 1e-9, L * torch.log(v2 / v1) / (0.6 * safe), L / v1)
+
+    ts = table(srcs)
+    tr = table(recs)
+    refl = np.zeros((ny, nx, nz))
+    for _ in range(6):
+        iz = rng.integers(10, nz - 5)
+        refl[:, :, iz] = rng.uniform(-1, 1)
+    refl += 0.01 * rng.standard_normal(refl.shape)
+    kw = dict(z=z, x=x, y=y, t=t, srcs=srcs, recs=recs, vel=1800.0,
+              wav=wav.astype(np.float32), wavcenter=wc, mode="byot", trav=(ts, tr))
+    desc = dict(workload="3d-byot-vz-128x128x120-25x400-nt1001", nx=nx, ny=ny, nz=nz, ns=25,
+                nr=400, nt=1001, dynamic=False)
+    return kw, refl.reshape(ny * nx, nz), desc
+
+
+def workload_2dsmall(rng):
+    x = np.arange(101) * 10.0
+    z = np.arange(100) * 10.0
+    t = np.arange(651) * 0.004
+    srcs = np.vstack((np.linspace(x[0], x[-1], 11), np.zeros(11)))
+    recs = np.vstack((np.linspace(x[0], x[-1], 51), np.zeros(51)))
+    wav, wc = ricker(t[:41], 20.0)
+    refl = 0.01 * rng.standard_normal((101, 100))
+    for iz in rng.integers(10, 90, 3):
+        refl[:, iz] += rng.uniform(-1, 1)
+    kw = dict(z=z, x=x, t=t, srcs=srcs, recs=recs, vel=2000.0, wav=wav.astype(np.float32),
+              wavcenter=wc, mode="analytic", dynamic=False)
+    desc = dict(workload="2d-kinematic-101x100-11x51-nt651", nx=101, nz=100, ns=11, nr=51,
+                nt=651, dynamic=False)
+    return kw, refl, desc
+
+
+def algorithmic_bytes(desc, ni, table_bytes, dtype_bytes=4):
+    """Separable-stream model: the receiver table streamed once per source,
+    source tables once, the image once and the data once; dynamic tables
+    (traveltime, two weights) triple the table terms, plus the velocity."""
+    ns, nr, nt = desc["ns"], desc["nr"], desc["nt"]
+    nk = 3 if desc["dynamic"] else 1
+    b = nk * (ns * nr * ni * table_bytes + ns * ni * table_bytes)
+    b += ni * dtype_bytes + ns * nr * nt * dtype_bytes
+    if desc["dynamic"]:
+        b += ni * dtype_bytes
+    return b
+
+
+def cpu_baseline(kw, desc, m, d, budget_s=12.0):
+    """The oracle (C/OpenMP restatement of the reference kernels, float32
+    operator arithmetic) on a bounded sample: the full image and receiver
+    set with a subset of the sources, forward + adjoint."""
+    from oracle import oracle as O
+    import torch
+    kwo = {k: v for k, v in kw.items()}
+    if kwo.get("mode") == "byot":
+        kwo["trav"] = tuple(a.cpu().numpy() if isinstance(a, torch.Tensor) else a
+                            for a in kwo["trav"])
+    kwo["dtype"] = "float32"
+    nsrc_total = kwo["srcs"].shape[1]
+    nthreads = O.lib().or_num_threads()
+
+    def run(nsrc):
+        k2 = dict(kwo)
+        k2["srcs"] = kwo["srcs"][:, :nsrc]
+        if k2.get("mode") == "byot":
+            k2["trav"] = (kwo["trav"][0][:, :nsrc], kwo["trav"][1])
+        orc = O.KirchhoffOracle(**k2)
+        dd = np.asarray(d).reshape(nsrc_total, -1)[:nsrc].ravel()
+        t0 = time.perf_counter()
+        orc.matvec(np.asarray(m, dtype=np.float32).ravel())
+        orc.rmatvec(dd)
+        return time.perf_counter() - t0, orc
+
+    n1 = min(nthreads, nsrc_total)
+    dt1, orc = run(n1)
+    nsrc = n1
+    if dt1 < budget_s / 2:
+        nsrc = int(min(nsrc_total, max(n1, n1 * budget_s / max(dt1, 1e-3))))
+        nsrc = max(n1, (nsrc // n1) * n1)
+        if nsrc != n1:
+            dt1, orc = run(nsrc)
+    trip = 2.0 * orc.ni * nsrc * orc.nr
+    return dict(value=trip / dt1 / 1e9, unit="Gsamples/s", cores=int(nthreads), kind="port",
+                sample=f"{desc['workload']} with {nsrc} of {nsrc_total} sources, "
+                       f"one forward + one adjoint (float32 operator arithmetic), "
+                       f"{dt1:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "2dsmall"])
+    ap.add_argument("--tables", default="f32", choices=["f32", "f16"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extra", action="store_true",
+                    help="also time the 3D workload with float32 and float16 tables")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+
+    import torch
+    import pylops_amd
+    from pylops_amd import _lib
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(1002)
+    if args.workload == "2d":
+        kw, refl, desc = workload_2d(rng)
+    elif args.workload == "3d":
+        kw, refl, desc = workload_3d(rng, dev)
+    else:
+        kw, refl, desc = workload_2dsmall(rng)
+    tdt = "float16" if args.tables == "f16" else None
+    Op = pylops_amd.Kirchhoff(dtype="float32", tabledtype=tdt, **kw)
+    plan = Op.plan
+    L = _lib.load()
+    import ctypes
+    pp = ctypes.byref(plan)
+    st = _lib.stream_ptr(dev)
+    mt = torch.as_tensor(np.asarray(refl, dtype=np.float32).ravel(), device=dev)
+    nd = Op.nsnr * Op.nt
+    dobs = torch.empty(nd, dtype=torch.float32, device=dev)
+    work = torch.empty(nd, dtype=torch.float32, device=dev)
+    dcorr = torch.empty(nd, dtype=torch.float32, device=dev)
+    img = torch.empty(Op.ni, dtype=torch.float32, device=dev)
+    _lib.check(L.kx_forward(pp, _lib.ptr(mt), _lib.ptr(dobs), _lib.ptr(work), st))
+    _lib.check(L.kx_adjoint(pp, _lib.ptr(dobs), _lib.ptr(img), _lib.ptr(work), st))
+    mdense = img.clone()
+    dout = torch.empty_like(dobs)
+    h, hs = Op._dev_tables["h"], Op._dev_tables["hstar"]
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        _lib.check(L.kx_spread(pp, _lib.ptr(mdense), _lib.ptr(work), st))
+        if ev is not None:
+            ev[1].record()
+        _lib.check(L.kx_conv(_lib.ptr(work), _lib.ptr(dout), Op.nsnr, Op.nt, _lib.ptr(h),
+                             h.numel(), _lib.KX_F32, st))
+        _lib.check(L.kx_conv(_lib.ptr(dobs), _lib.ptr(dcorr), Op.nsnr, Op.nt, _lib.ptr(hs),
+                             hs.numel(), _lib.KX_F32, st))
+        if ev is not None:
+            ev[2].record()
+        _lib.check(L.kx_gather(pp, _lib.ptr(dcorr), _lib.ptr(img), st))
+        if ev is not None:
+            ev[3].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_spread = np.mean([e[0].elapsed_time(e[1]) for e in evs]) * 1e-3
+    t_conv = np.mean([e[1].elapsed_time(e[2]) for e in evs]) * 1e-3
+    t_gather = np.mean([e[2].elapsed_time(e[3]) for e in evs]) * 1e-3
+    ms_per_step = wall / args.steps * 1e3
+    trip = float(Op.ni) * Op.ns * Op.nr
+    value = 2.0 * trip / (wall / args.steps) / 1e9
+
+    tb = 2 if args.tables == "f16" else 4
+    bytes_alg = algorithmic_bytes(desc, Op.ni, tb)
+    dom, t_dom = ("gather", t_gather) if t_gather >= t_spread else ("spread", t_spread)
+    achieved = bytes_alg / t_dom / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {**desc, "tables": args.tables, "step": "forward of K^T d + adjoint of d",
+                   "kernel_ms": {"spread": round(t_spread * 1e3, 4),
+                                 "conv_fwd+adj": round(t_conv * 1e3, 4),
+                                 "gather": round(t_gather * 1e3, 4)}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "model_bytes": int(bytes_alg)},
+    }
+    if args.extra:
+        out["extra"] = extra_3d(dev)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(kw, desc, refl, dobs.cpu().numpy())
+    print(json.dumps(out), flush=True)
+
+
+def extra_3d(dev):
+    """3D workload: fp32 vs fp16 tables, forward+adjoint time and error of
+    the fp16 result against the fp32 result."""
+    import torch
+    import pylops_amd
+    rng = np.random.default_rng(1003)
+    kw, refl, desc = workload_3d(rng, dev)
+    res = {}
+    m = torch.as_tensor(refl.astype(np.float32).ravel(), device=dev)
+    outs = {}
+    for tabs in ("f32", "f16"):
+        Op = pylops_amd.Kirchhoff(dtype="float32", tabledtype="float16" if tabs == "f16" else None,
+                                  **kw)
+        d = Op @ m
+        a = Op.H @ d
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = 3
+        for _ in range(n):
+            d = Op @ m
+            a = Op.H @ d
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / n
+        outs[tabs] = (d, a)
+        res[tabs] = {"ms_fwd_adj": round(dt * 1e3, 3),
+                     "gsamples_s": round(2 * Op.ni * Op.ns * Op.nr / dt / 1e9, 2)}
+        del Op
+    d32, a32 = outs["f32"]
+    d16, a16 = outs["f16"]
+    res["fp16_rel_err_fwd_vs_fp32"] = float((d16 - d32).norm() / d32.norm())
+    res["fp16_speedup"] = round(res["f32"]["ms_fwd_adj"] / res["f16"]["ms_fwd_adj"], 3)
+    res["workload"] = desc["workload"]
+    return res
+
+
+if __name__ == "__main__":
+    main()

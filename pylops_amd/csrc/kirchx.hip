@@ -76,6 +76,11 @@ constexpr int kReduceBlocks = 1024;
 template <typename TD> struct TimeType { using type = float; };
 template <> struct TimeType<double> { using type = double; };
 
+// LDS trace accumulator of the forward.  float64: on gfx950 an LDS float64
+// atomic add sustains ~3 lanes/clk/CU against ~0.3 for float32, and the
+// traces come out more accurate.
+using AccT = double;
+
 __device__ __forceinline__ float ld_off(const float *p) { return *p; }
 __device__ __forceinline__ float ld_off(const __half *p) { return __half2float(*p); }
 __device__ __forceinline__ double ld_off(const double *p) { return *p; }
@@ -136,85 +141,153 @@ __device__ __forceinline__ int64_t point_index(const KArgs &a, const TilePoint &
 // ---------------------------------------------------------------------------
 // forward: spread into LDS-resident traces
 // ---------------------------------------------------------------------------
-template <typename TD, typename TT, bool DYN, bool NEG>
+// Everything one tile of one wave needs: the image value and the source /
+// receiver offsets, bases and weights of its 64 points.
+template <typename TD, typename TC, int RB> struct SpreadTile {
+    TD mv;
+    TC os;
+    int bs;
+    TC orr[RB];
+    int br[RB];
+    TD ps, qs;
+    TD pr[RB], qr[RB];
+    int iz;
+};
+
+// Opaque zero held in a VGPR: indexing with it keeps a wave-uniform load on
+// the vector memory path.  Scalar loads share the lgkm counter with LDS
+// operations, so waiting on a scalar load would also wait for every LDS
+// atomic still in flight.
+__device__ __forceinline__ int vgpr_zero()
+{
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    return z;
+}
+
+template <typename TD, typename TT, bool DYN, int RB>
+__device__ __forceinline__ void spread_load(const KArgs &a, int t, int lane, int nrl, int vz,
+                                            const TD *__restrict__ m, const TT *offs,
+                                            const int *bss, const TT *offr, const int *bsr,
+                                            const TD *wps, const TD *wqs, const TD *wpr,
+                                            const TD *wqr, size_t rstride,
+                                            SpreadTile<TD, typename TimeType<TD>::type, RB> &L)
+{
+    using TC = typename TimeType<TD>::type;
+    const TilePoint p = tile_point(a, t, lane);
+    L.iz = p.iz;
+    L.mv = p.valid ? m[point_index(a, p)] : TD(0);
+    const size_t tl = (size_t)t * kWave + lane;
+    L.os = (TC)ld_off(offs + tl);
+    L.bs = bss[t + vz];
+    if constexpr (DYN) {
+        L.ps = wps[tl];
+        L.qs = wqs[tl];
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+        if (j < nrl) {
+            L.orr[j] = (TC)ld_off(offr + j * rstride + tl);
+            L.br[j] = bsr[(size_t)j * a.ntiles + t + vz];
+            if constexpr (DYN) {
+                L.pr[j] = wpr[j * rstride + tl];
+                L.qr[j] = wqr[j * rstride + tl];
+            }
+        }
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, int RB>
 __global__ __launch_bounds__(kBlock) void spread_kernel(KArgs a, const TD *__restrict__ m,
                                                         TD *__restrict__ y)
 {
     using TC = typename TimeType<TD>::type;
     extern __shared__ __align__(16) unsigned char kx_smem[];
-    TD *acc = reinterpret_cast<TD *>(kx_smem);
+    AccT *acc = reinterpret_cast<AccT *>(kx_smem);
 
-    const int nrb = (a.nr + a.rblock - 1) / a.rblock;
+    // one workgroup = one source s and receivers [r0, r0 + RB)
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int s = L % a.ns;
-    const int r0 = (L / a.ns) * a.rblock;
-    if (r0 >= a.nr || L >= a.ns * nrb)
-        return;  // uniform for the whole workgroup
-    const int nrl = min(a.rblock, a.nr - r0);
+    const int r0 = (L / a.ns) * RB;
+    const int nrl = min(RB, a.nr - r0);
     const int ntr = nrl * a.nt;
 
     for (int i = threadIdx.x; i < ntr; i += blockDim.x)
-        acc[i] = TD(0);
+        acc[i] = AccT(0);
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int nw = blockDim.x / kWave;
-    const TT *offs = reinterpret_cast<const TT *>(a.off_s) + (size_t)s * a.ntiles * kWave;
-    const TT *offr = reinterpret_cast<const TT *>(a.off_r);
+    const int vz = vgpr_zero();
+    const size_t rstride = (size_t)a.ntiles * kWave;  // next receiver row
+    const TT *offs = reinterpret_cast<const TT *>(a.off_s) + (size_t)s * rstride;
+    const TT *offr = reinterpret_cast<const TT *>(a.off_r) + (size_t)r0 * rstride;
     const int *bss = a.base_s + (size_t)s * a.ntiles;
-    const TD *wps = reinterpret_cast<const TD *>(a.wp_s);
-    const TD *wqs = reinterpret_cast<const TD *>(a.wq_s);
-    const TD *wpr = reinterpret_cast<const TD *>(a.wp_r);
-    const TD *wqr = reinterpret_cast<const TD *>(a.wq_r);
+    const int *bsr = a.base_r + (size_t)r0 * a.ntiles;
+    const TD *wps = reinterpret_cast<const TD *>(a.wp_s) + (size_t)s * rstride;
+    const TD *wqs = reinterpret_cast<const TD *>(a.wq_s) + (size_t)s * rstride;
+    const TD *wpr = reinterpret_cast<const TD *>(a.wp_r) + (size_t)r0 * rstride;
+    const TD *wqr = reinterpret_cast<const TD *>(a.wq_r) + (size_t)r0 * rstride;
 
-    for (int t = wave; t < a.ntiles; t += nw) {
-        const TilePoint p = tile_point(a, t, lane);
-        const TD mv = p.valid ? m[point_index(a, p)] : TD(0);
-        if (__ballot(mv != TD(0)) == 0)
-            continue;  // the whole tile is empty
-        const TC os = (TC)ld_off(offs + (size_t)t * kWave + lane);
-        const int bs = bss[t];
-        TD ps = TD(0), qs = TD(0);
-        if constexpr (DYN) {
-            const size_t so = ((size_t)s * a.ntiles + t) * kWave + lane;
-            ps = wps[so];
-            qs = wqs[so];
-        }
-        for (int j = 0; j < nrl; ++j) {
-            const int r = r0 + j;
-            const size_t ro = (size_t)r * a.ntiles + t;
-            const TC orr = (TC)ld_off(offr + ro * kWave + lane);
-            const int br = a.base_r[ro];
-            const TC T = os + orr;
-            const TC fl = floor(T);
-            TC f = T - fl;
-            int it = (int)fl + bs + br;
-            if constexpr (NEG) {
-                // the reference truncates toward zero: a time in (-1, 0)
-                // lands on sample 0 with a negative weight
-                if (it == -1 && f > TC(0)) {
-                    it = 0;
-                    f -= TC(1);
+    int izmin[RB];
+    if constexpr (DYN) {
+#pragma unroll
+        for (int j = 0; j < RB; ++j)
+            izmin[j] = j < nrl ? a.izmin[s * a.nr + r0 + j] : 0;
+    }
+
+    SpreadTile<TD, TC, RB> cur, nxt;
+    int t = wave;
+    if (t < a.ntiles)
+        spread_load<TD, TT, DYN, RB>(a, t, lane, nrl, vz, m, offs, bss, offr, bsr, wps, wqs,
+                                     wpr, wqr, rstride, cur);
+    for (; t < a.ntiles; t += nw) {
+        // prefetch the next tile while this one is accumulated
+        if (t + nw < a.ntiles)
+            spread_load<TD, TT, DYN, RB>(a, t + nw, lane, nrl, vz, m, offs, bss, offr, bsr,
+                                         wps, wqs, wpr, wqr, rstride, nxt);
+        if (__ballot(cur.mv != TD(0)) != 0) {
+            int it[RB];
+            AccT w0[RB], w1[RB];
+#pragma unroll
+            for (int j = 0; j < RB; ++j) {
+                const TC T = cur.os + cur.orr[j];
+                const TC fl = floor(T);
+                TC f = T - fl;
+                int k = (int)fl + cur.bs + cur.br[j];
+                if constexpr (NEG) {
+                    // the reference truncates toward zero: a time in (-1, 0)
+                    // lands on sample 0 with a negative weight
+                    if (k == -1 && f > TC(0)) {
+                        k = 0;
+                        f -= TC(1);
+                    }
+                }
+                TD w = cur.mv;
+                if constexpr (DYN) {
+                    w = cur.mv * (cur.ps * cur.pr[j] + cur.qs * cur.qr[j]);
+                    if (cur.iz < izmin[j])
+                        w = TD(0);
+                }
+                it[j] = (j < nrl && (unsigned)k < (unsigned)(a.nt - 1)) ? j * a.nt + k : -1;
+                w0[j] = (AccT)(w * (TD)(TC(1) - f));
+                w1[j] = (AccT)(w * (TD)f);
+            }
+#pragma unroll
+            for (int j = 0; j < RB; ++j) {
+                if (it[j] >= 0) {
+                    atomicAdd(&acc[it[j]], w0[j]);
+                    atomicAdd(&acc[it[j] + 1], w1[j]);
                 }
             }
-            TD w = mv;
-            if constexpr (DYN) {
-                const size_t rl = ro * kWave + lane;
-                w = mv * (ps * wpr[rl] + qs * wqr[rl]);
-                if (p.iz < a.izmin[s * a.nr + r])
-                    w = TD(0);
-            }
-            if ((unsigned)it < (unsigned)(a.nt - 1)) {
-                atomicAdd(&acc[j * a.nt + it], w * (TD)(TC(1) - f));
-                atomicAdd(&acc[j * a.nt + it + 1], w * (TD)f);
-            }
         }
+        cur = nxt;
     }
     __syncthreads();
     TD *yo = y + ((size_t)s * a.nr + r0) * a.nt;
     for (int i = threadIdx.x; i < ntr; i += blockDim.x)
-        yo[i] = acc[i];
+        yo[i] = (TD)acc[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -514,28 +587,55 @@ KArgs make_args(const kx_plan *p)
     return a;
 }
 
+template <typename TD, typename TT, bool DYN, bool NEG, int RB>
+int launch_spread_rb(const KArgs &a, const void *m, void *y, hipStream_t st)
+{
+    const size_t lds = (size_t)RB * a.nt * sizeof(AccT);
+    const int nrb = (a.nr + RB - 1) / RB;
+    const int nblk = a.ns * nrb;
+    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, RB>), dim3(nblk), dim3(kBlock), lds, st,
+                       a, reinterpret_cast<const TD *>(m), reinterpret_cast<TD *>(y));
+    KX_LAUNCHED("spread");
+    return KX_OK;
+}
+
 template <typename TD, typename TT, bool DYN, bool NEG>
 int launch_spread_t(const KArgs &a, const void *m, void *y, hipStream_t st)
 {
-    const size_t lds = (size_t)a.rblock * a.nt * sizeof(TD);
-    const int nrb = (a.nr + a.rblock - 1) / a.rblock;
-    const int nblk = a.ns * nrb;
-    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG>), dim3(nblk), dim3(kBlock), lds, st, a,
-                       reinterpret_cast<const TD *>(m), reinterpret_cast<TD *>(y));
-    KX_LAUNCHED("spread");
-    return KX_OK;
+    switch (a.rblock) {
+    case 16:
+        return launch_spread_rb<TD, TT, DYN, NEG, 16>(a, m, y, st);
+    case 8:
+        return launch_spread_rb<TD, TT, DYN, NEG, 8>(a, m, y, st);
+    case 4:
+        return launch_spread_rb<TD, TT, DYN, NEG, 4>(a, m, y, st);
+    case 2:
+        return launch_spread_rb<TD, TT, DYN, NEG, 2>(a, m, y, st);
+    default:
+        return launch_spread_rb<TD, TT, DYN, NEG, 1>(a, m, y, st);
+    }
+}
+
+// receivers per workgroup: a power of two in [1, 16] whose traces fit in LDS
+int pick_rblock(int requested, int nr, int per_trace_bytes)
+{
+    int rb = requested > 0 ? requested : 8;
+    rb = std::min(rb, kLdsBudget / per_trace_bytes);
+    int p2 = 1;
+    while (p2 * 2 <= rb && p2 * 2 <= 16)
+        p2 *= 2;
+    while (p2 > 1 && p2 / 2 >= nr)
+        p2 /= 2;  // do not leave a block mostly empty
+    return p2;
 }
 
 template <typename TD, typename TT>
 int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream_t st)
 {
-    const int per_trace = p->nt * (int)sizeof(TD);
+    const int per_trace = p->nt * (int)sizeof(AccT);
     if (per_trace > kLdsBudget)
         return kx_fail(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
-    int rb = p->rblock > 0 ? p->rblock : 8;
-    rb = std::min(rb, kLdsBudget / per_trace);
-    rb = std::min(rb, p->nr);
-    a.rblock = std::max(rb, 1);
+    a.rblock = pick_rblock(p->rblock, p->nr, per_trace);
     if (p->dynamic)
         return p->neg_trav ? launch_spread_t<TD, TT, true, true>(a, m, y, st)
                            : launch_spread_t<TD, TT, true, false>(a, m, y, st);

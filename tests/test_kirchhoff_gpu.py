@@ -186,7 +186,6 @@ def test_conv_matches_oracle():
 
 
 def test_dot_fp64_accumulation():
-    import ctypes
     import torch
     from pylops_amd import _lib
     L = _lib.load()
@@ -203,4 +202,3 @@ def test_dot_fp64_accumulation():
         _lib.check(rc)
         ref = np.dot(a.astype(np.float64), b.astype(np.float64))
         assert abs(float(out) - ref) <= 1e-12 * max(1.0, np.abs(a).dot(np.abs(b)))
-        del ctypes
