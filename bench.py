@@ -214,6 +214,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true",
                     help="also time the 3D workload with float32 and float16 tables")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 4 GiB device copy first (reference kernel for "
+                         "FETCH_SIZE / WRITE_SIZE counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -226,6 +229,12 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.calib_copy:
+        src = torch.ones(1024 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     rng = np.random.default_rng(1002)
     if args.workload == "2d":
         kw, refl, desc = workload_2d(rng)

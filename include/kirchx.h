@@ -26,8 +26,9 @@
  *   kx_encode_table / kx_tile_table
  *               <- the table setup of Kirchhoff.__init__ K:304-398
  *                  (device layout only; values come from the caller)
- *   kx_dot, kx_cgls_*, kx_lsqr_* <- the vector algebra of CGLS.step
- *                  CB:510-564 and LSQR.step CB:987-1136
+ *   kx_dot, kx_dot_dev, kx_vec, kx_lsqr_xw, kx_cgls_scalar, kx_lsqr_scalar
+ *               <- the vector algebra and host scalar updates of CGLS.step
+ *                  CB:510-564 and LSQR.step CB:987-1136 (float64 reductions)
  */
 #ifndef KIRCHX_H
 #define KIRCHX_H
@@ -124,11 +125,104 @@ int kx_forward(const kx_plan *p, const void *m, void *d, void *work, void *strea
 int kx_adjoint(const kx_plan *p, const void *d, void *m, void *work, void *stream);
 
 /* ---- vector algebra for the solvers (float64 accumulation) ---- */
-/* Bytes of workspace for the reductions below. */
+/* Bytes of workspace for kx_dot. */
 int64_t kx_reduce_workspace_bytes(void);
 /* out[0] = sum a[i]*b[i] accumulated in float64; out is a device double. */
 int kx_dot(const void *a, const void *b, int64_t n, int32_t dtype, double *out,
            void *work, void *stream);
+
+/* Solver scalars live in a device float64 array `sc`.
+ * CGLS layout (CGLS.step, CB:510-564): */
+#define KX_CG_KOLD 0
+#define KX_CG_QQ 1
+#define KX_CG_CC 2
+#define KX_CG_A 3
+#define KX_CG_K 4
+#define KX_CG_B 5
+#define KX_CG_SS 6
+#define KX_CG_XX 7
+#define KX_CG_DAMP2 8
+#define KX_CG_TOL 9
+#define KX_CG_ACTIVE 10
+#define KX_CG_ITER 11
+#define KX_CG_NITER 12
+#define KX_CG_NSC 16
+/* LSQR layout (LSQR.step, CB:987-1136): */
+#define KX_LS_ALFA 0
+#define KX_LS_BETA 1
+#define KX_LS_ANORM 2
+#define KX_LS_ACOND 3
+#define KX_LS_DAMP 4
+#define KX_LS_DAMPSQ 5
+#define KX_LS_DDNORM 6
+#define KX_LS_RES2 7
+#define KX_LS_XNORM 8
+#define KX_LS_XXNORM 9
+#define KX_LS_Z 10
+#define KX_LS_CS2 11
+#define KX_LS_SN2 12
+#define KX_LS_RHOBAR 13
+#define KX_LS_PHIBAR 14
+#define KX_LS_BNORM 15
+#define KX_LS_RNORM 16
+#define KX_LS_R1NORM 17
+#define KX_LS_R2NORM 18
+#define KX_LS_ARNORM 19
+#define KX_LS_ARNORM0 20
+#define KX_LS_TEST1 21
+#define KX_LS_TEST2 22
+#define KX_LS_TEST3 23
+#define KX_LS_ATOL 24
+#define KX_LS_BTOL 25
+#define KX_LS_CTOL 26
+#define KX_LS_ISTOP 27
+#define KX_LS_ITER 28
+#define KX_LS_NITER 29
+#define KX_LS_ACTIVE 30
+#define KX_LS_T1 31
+#define KX_LS_T2 32
+#define KX_LS_RHO 33
+#define KX_LS_THETA 34
+#define KX_LS_PHI 35
+#define KX_LS_TAU 36
+#define KX_LS_PSI 37
+#define KX_LS_UU 38
+#define KX_LS_VV 39
+#define KX_LS_WW 40
+#define KX_LS_VAR 41
+#define KX_LS_RTOL 42
+#define KX_LS_BETAPOS 43
+#define KX_LS_ALFAPOS 44
+#define KX_LS_NSC 48
+
+/* kx_vec operations (c = sign * coef[0], or 1 / coef[0] for SCALE):
+ *   KX_VEC_XPBY   y = x + c y
+ *   KX_VEC_AXPY   y = y + c x
+ *   KX_VEC_SCALE  y = y / coef[0]
+ * and, when out != NULL, out[0] = y.y in float64.  The kernel does nothing
+ * unless every non-NULL gate points at a non-zero value. */
+#define KX_VEC_XPBY 0
+#define KX_VEC_AXPY 1
+#define KX_VEC_SCALE 2
+
+/* Bytes of the reduction workspace of the calls below; zero it once after
+ * allocation, the kernels leave it zeroed. */
+int64_t kx_solver_workspace_bytes(void);
+int kx_vec(int32_t op, int64_t n, int32_t dtype, const void *x, void *y,
+           const double *coef, double sign, double *out, void *work,
+           const double *gate0, const double *gate1, void *stream);
+/* out[0] = a.b in float64, skipped when *gate0 == 0. */
+int kx_dot_dev(int64_t n, int32_t dtype, const void *a, const void *b, double *out,
+               void *work, const double *gate0, void *stream);
+/* LSQR x / w update: sc[KX_LS_WW] = w.w, x += t1 w, w = v + t2 w. */
+int kx_lsqr_xw(int64_t n, int32_t dtype, void *x, void *w, const void *v, double *sc,
+               void *work, void *stream);
+/* Scalar recurrences, one thread: CGLS phases 0 (step size a), 1 (beta),
+ * 2 (end of iteration: kold, counters, cost[iter], cost1[iter], loop flag);
+ * LSQR phases 0 (beta, anorm), 1 (alfa), 2 (rotations), 3 (norms, tests,
+ * cost[iter], loop flag). */
+int kx_cgls_scalar(double *sc, double *cost, double *cost1, int32_t phase, void *stream);
+int kx_lsqr_scalar(double *sc, double *cost, int32_t phase, void *stream);
 
 #ifdef __cplusplus
 }

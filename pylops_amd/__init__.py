@@ -12,6 +12,7 @@ from .linearoperator import LinearOperator, aslinearoperator  # noqa: F401
 from . import utils  # noqa: F401
 from . import signalprocessing  # noqa: F401
 from . import waveeqprocessing  # noqa: F401
+from . import optimization  # noqa: F401
 from .signalprocessing import Convolve1D  # noqa: F401
 from .waveeqprocessing import Kirchhoff  # noqa: F401
 

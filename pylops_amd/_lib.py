@@ -20,7 +20,75 @@ EXPORTS = (
     "kx_version", "kx_last_error", "kx_workspace_bytes", "kx_encode_table",
     "kx_tile_table", "kx_spread", "kx_gather", "kx_conv", "kx_forward",
     "kx_adjoint", "kx_reduce_workspace_bytes", "kx_dot",
+    "kx_solver_workspace_bytes", "kx_vec", "kx_dot_dev", "kx_lsqr_xw",
+    "kx_cgls_scalar", "kx_lsqr_scalar",
 )
+
+# solver scalar layouts and vector ops (include/kirchx.h)
+KX_CG_KOLD = 0
+KX_CG_QQ = 1
+KX_CG_CC = 2
+KX_CG_A = 3
+KX_CG_K = 4
+KX_CG_B = 5
+KX_CG_SS = 6
+KX_CG_XX = 7
+KX_CG_DAMP2 = 8
+KX_CG_TOL = 9
+KX_CG_ACTIVE = 10
+KX_CG_ITER = 11
+KX_CG_NITER = 12
+KX_CG_NSC = 16
+KX_LS_ALFA = 0
+KX_LS_BETA = 1
+KX_LS_ANORM = 2
+KX_LS_ACOND = 3
+KX_LS_DAMP = 4
+KX_LS_DAMPSQ = 5
+KX_LS_DDNORM = 6
+KX_LS_RES2 = 7
+KX_LS_XNORM = 8
+KX_LS_XXNORM = 9
+KX_LS_Z = 10
+KX_LS_CS2 = 11
+KX_LS_SN2 = 12
+KX_LS_RHOBAR = 13
+KX_LS_PHIBAR = 14
+KX_LS_BNORM = 15
+KX_LS_RNORM = 16
+KX_LS_R1NORM = 17
+KX_LS_R2NORM = 18
+KX_LS_ARNORM = 19
+KX_LS_ARNORM0 = 20
+KX_LS_TEST1 = 21
+KX_LS_TEST2 = 22
+KX_LS_TEST3 = 23
+KX_LS_ATOL = 24
+KX_LS_BTOL = 25
+KX_LS_CTOL = 26
+KX_LS_ISTOP = 27
+KX_LS_ITER = 28
+KX_LS_NITER = 29
+KX_LS_ACTIVE = 30
+KX_LS_T1 = 31
+KX_LS_T2 = 32
+KX_LS_RHO = 33
+KX_LS_THETA = 34
+KX_LS_PHI = 35
+KX_LS_TAU = 36
+KX_LS_PSI = 37
+KX_LS_UU = 38
+KX_LS_VV = 39
+KX_LS_WW = 40
+KX_LS_VAR = 41
+KX_LS_RTOL = 42
+KX_LS_BETAPOS = 43
+KX_LS_ALFAPOS = 44
+KX_LS_NSC = 48
+KX_VEC_XPBY = 0
+KX_VEC_AXPY = 1
+KX_VEC_SCALE = 2
+
 
 
 class KxPlan(ctypes.Structure):
@@ -74,6 +142,12 @@ def _declare(L):
         "kx_adjoint": ([PP, P, P, P, P], ctypes.c_int),
         "kx_reduce_workspace_bytes": ([], I64),
         "kx_dot": ([P, P, I64, I32, P, P, P], ctypes.c_int),
+        "kx_solver_workspace_bytes": ([], I64),
+        "kx_vec": ([I32, I64, I32, P, P, P, D, P, P, P, P, P], ctypes.c_int),
+        "kx_dot_dev": ([I64, I32, P, P, P, P, P, P], ctypes.c_int),
+        "kx_lsqr_xw": ([I64, I32, P, P, P, P, P, P], ctypes.c_int),
+        "kx_cgls_scalar": ([P, P, P, I32, P], ctypes.c_int),
+        "kx_lsqr_scalar": ([P, P, I32, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -60,6 +60,12 @@ static int kx_fail(int code, const char *fmt, ...)
             return kx_fail((int)e_, "%s launch: %s", name, hipGetErrorString(e_)); \
     } while (0)
 
+// error hook for the other translation units of the library (not exported)
+extern "C" __attribute__((visibility("hidden"))) int kx_solver_fail(int code, const char *msg)
+{
+    return kx_fail(code, "%s", msg);
+}
+
 extern "C" int kx_version(void) { return KX_ABI_VERSION; }
 extern "C" const char *kx_last_error(void) { return g_err; }
 
@@ -143,11 +149,11 @@ __device__ __forceinline__ int64_t point_index(const KArgs &a, const TilePoint &
 // ---------------------------------------------------------------------------
 // Everything one tile of one wave needs: the image value and the source /
 // receiver offsets, bases and weights of its 64 points.
-template <typename TD, typename TC, int RB> struct SpreadTile {
+template <typename TD, typename TT, int RB> struct SpreadTile {
     TD mv;
-    TC os;
+    TT os;  // raw table elements: converted at use, so a prefetch never waits
     int bs;
-    TC orr[RB];
+    TT orr[RB];
     int br[RB];
     TD ps, qs;
     TD pr[RB], qr[RB];
@@ -171,14 +177,13 @@ __device__ __forceinline__ void spread_load(const KArgs &a, int t, int lane, int
                                             const int *bss, const TT *offr, const int *bsr,
                                             const TD *wps, const TD *wqs, const TD *wpr,
                                             const TD *wqr, size_t rstride,
-                                            SpreadTile<TD, typename TimeType<TD>::type, RB> &L)
+                                            SpreadTile<TD, TT, RB> &L)
 {
-    using TC = typename TimeType<TD>::type;
     const TilePoint p = tile_point(a, t, lane);
     L.iz = p.iz;
     L.mv = p.valid ? m[point_index(a, p)] : TD(0);
     const size_t tl = (size_t)t * kWave + lane;
-    L.os = (TC)ld_off(offs + tl);
+    L.os = offs[tl];
     L.bs = bss[t + vz];
     if constexpr (DYN) {
         L.ps = wps[tl];
@@ -187,7 +192,7 @@ __device__ __forceinline__ void spread_load(const KArgs &a, int t, int lane, int
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
         if (j < nrl) {
-            L.orr[j] = (TC)ld_off(offr + j * rstride + tl);
+            L.orr[j] = offr[j * rstride + tl];
             L.br[j] = bsr[(size_t)j * a.ntiles + t + vz];
             if constexpr (DYN) {
                 L.pr[j] = wpr[j * rstride + tl];
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void spread_kernel(KArgs a, const TD *__res
             izmin[j] = j < nrl ? a.izmin[s * a.nr + r0 + j] : 0;
     }
 
-    SpreadTile<TD, TC, RB> cur, nxt;
+    SpreadTile<TD, TT, RB> cur, nxt;
     int t = wave;
     if (t < a.ntiles)
         spread_load<TD, TT, DYN, RB>(a, t, lane, nrl, vz, m, offs, bss, offr, bsr, wps, wqs,
@@ -250,9 +255,10 @@ __global__ __launch_bounds__(kBlock) void spread_kernel(KArgs a, const TD *__res
         if (__ballot(cur.mv != TD(0)) != 0) {
             int it[RB];
             AccT w0[RB], w1[RB];
+            const TC os = (TC)ld_off(&cur.os);
 #pragma unroll
             for (int j = 0; j < RB; ++j) {
-                const TC T = cur.os + cur.orr[j];
+                const TC T = os + (TC)ld_off(&cur.orr[j]);
                 const TC fl = floor(T);
                 TC f = T - fl;
                 int k = (int)fl + cur.bs + cur.br[j];

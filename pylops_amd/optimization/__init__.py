@@ -1,0 +1,4 @@
+from . import basic  # noqa: F401
+from .basic import cgls, lsqr  # noqa: F401
+from .callback import Callbacks  # noqa: F401
+from .cls_basic import CGLS, LSQR  # noqa: F401

@@ -1,0 +1,93 @@
+"""Solver template with callback hooks (pylops/optimization/basesolver.py:18-330)."""
+import functools
+import time
+from abc import ABCMeta, abstractmethod
+
+from pylops_amd.optimization.callback import Callbacks  # noqa: F401
+
+__all__ = ["Solver"]
+
+_units = {"B": 1, "KB": 1024, "MB": 1024 ** 2, "GB": 1024 ** 3}
+
+
+class Solver(metaclass=ABCMeta):
+    """Base class: ``setup``, ``step``, ``run``, ``finalize``, ``solve`` and
+    an optional ``callback(x)`` called after every step.  ``callbacks`` is a
+    list of :class:`Callbacks` whose ``on_setup/step/run_begin/end`` hooks
+    wrap the corresponding methods."""
+
+    def __init__(self, Op, callbacks=None):
+        self.Op = Op
+        self.callbacks = callbacks
+        self._registercallbacks()
+        self.iiter = 0
+        self.tstart = time.time()
+
+    def _print_solver(self, text="", nbar=80):
+        print(f"{type(self).__name__}" + text)
+        print("-" * nbar + "\n"
+              f"The Operator Op has {self.Op.shape[0]} rows and {self.Op.shape[1]} cols")
+
+    def _print_setup(self, *args, **kwargs):
+        pass
+
+    def _print_step(self, *args, **kwargs):
+        pass
+
+    def _print_finalize(self, *args, nbar=80, **kwargs):
+        print(f"\nIterations = {self.iiter}        Total time (s) = {self.telapsed:.2f}")
+        print("-" * nbar + "\n")
+
+    def _registercallbacks(self):
+        def cbdecorator(func, setup=False):
+            @functools.wraps(func)
+            def wrapper(*args, **kwargs):
+                if self.callbacks:
+                    for cb in self.callbacks:
+                        arg = kwargs.get("x0", None) if setup else args[0]
+                        getattr(cb, f"on_{func.__name__}_begin")(self, arg)
+                ret = func(*args, **kwargs)
+                if self.callbacks:
+                    for cb in self.callbacks[::-1]:
+                        arg = kwargs.get("x0", None) if setup else args[0]
+                        getattr(cb, f"on_{func.__name__}_end")(self, arg)
+                return ret
+            return wrapper
+
+        for method in ("setup", "step", "run"):
+            setattr(self, method, cbdecorator(getattr(self, method), method == "setup"))
+
+    def _wants_host_each_step(self):
+        """True when something outside the device needs every iteration:
+        a user callback, a callbacks list, or printing."""
+        return bool(self.callbacks) or "callback" in self.__dict__
+
+    @abstractmethod
+    def memory_usage(self, show=False, unit="B"):
+        pass
+
+    @abstractmethod
+    def setup(self, y, *args, show=False, **kwargs):
+        pass
+
+    @abstractmethod
+    def step(self, x, *args, show=False, **kwargs):
+        pass
+
+    @abstractmethod
+    def run(self, x, *args, show=False, **kwargs):
+        pass
+
+    def finalize(self, *args, show=False, **kwargs):
+        self.tend = time.time()
+        self.telapsed = self.tend - self.tstart
+        if show:
+            self._print_finalize()
+
+    @abstractmethod
+    def solve(self, y, *args, show=False, **kwargs):
+        pass
+
+    def callback(self, x, *args, **kwargs):
+        """Called after every step; replace to monitor the solver."""
+        pass

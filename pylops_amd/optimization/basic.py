@@ -1,0 +1,74 @@
+"""Function front-ends of the solvers (pylops/optimization/basic.py:89-332):
+same signatures and return tuples as the reference.  numpy inputs are moved
+to the operator's device and results come back as numpy; torch device
+tensors stay on the device."""
+import numpy as np
+import torch
+
+from pylops_amd.optimization.cls_basic import CGLS, LSQR
+from pylops_amd.utils.backend import is_torch, torch_dtype
+from pylops_amd.utils.decorators import add_ndarray_support_to_solver
+
+__all__ = ["cgls", "lsqr"]
+
+
+def _device_of(Op, y):
+    if is_torch(y) and y.is_cuda:
+        return y.device
+    dev = getattr(Op, "device", None)
+    return dev if dev is not None else torch.device("cuda", torch.cuda.current_device())
+
+
+def _to_dev(a, device, dtype):
+    if a is None:
+        return None
+    if is_torch(a):
+        return a.to(device=device, dtype=dtype).reshape(-1).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(a).ravel()), dtype=dtype,
+                           device=device)
+
+
+def _prepare(Op, y, x0, callback, solver):
+    from pylops_amd import _lib
+    _lib.require_device()
+    dev = _device_of(Op, y)
+    dt = y.dtype if is_torch(y) else torch_dtype(np.asarray(y).dtype)
+    numpy_io = not is_torch(y)
+    yd = _to_dev(y, dev, dt)
+    x0d = _to_dev(x0, dev, dt)
+    if callback is not None:
+        if numpy_io:
+            solver.callback = lambda x: callback(x.detach().cpu().numpy())
+        else:
+            solver.callback = callback
+    back = (lambda t: t.detach().cpu().numpy()) if numpy_io else (lambda t: t)
+    return yd, x0d, back
+
+
+@add_ndarray_support_to_solver
+def cgls(Op, y, x0=None, niter=10, damp=0.0, tol=1e-4, show=False, itershow=(10, 10, 10),
+         callback=None, preallocate=False):
+    """Conjugate gradient least squares; returns
+    ``(x, istop, iit, r1norm, r2norm, cost)`` as the reference."""
+    cgsolve = CGLS(Op)
+    yd, x0d, back = _prepare(Op, y, x0, callback, cgsolve)
+    x, istop, iiter, r1norm, r2norm, cost = cgsolve.solve(
+        y=yd, x0=x0d, tol=tol, niter=niter, damp=damp, show=show, itershow=itershow,
+        preallocate=preallocate)
+    return back(x), istop, iiter, r1norm, r2norm, cost
+
+
+@add_ndarray_support_to_solver
+def lsqr(Op, y, x0=None, damp=0.0, atol=1e-08, btol=1e-08, conlim=100000000.0, niter=10,
+         calc_var=True, show=False, itershow=(10, 10, 10), callback=None, preallocate=False):
+    """LSQR; returns ``(x, istop, iiter, r1norm, r2norm, anorm, acond,
+    arnorm, xnorm, var, cost)`` as the reference."""
+    lsqrsolve = LSQR(Op)
+    yd, x0d, back = _prepare(Op, y, x0, callback, lsqrsolve)
+    (x, istop, iiter, r1norm, r2norm, anorm, acond, arnorm, xnorm, var,
+     cost) = lsqrsolve.solve(y=yd, x0=x0d, damp=damp, atol=atol, btol=btol, conlim=conlim,
+                             niter=niter, calc_var=calc_var, show=show, itershow=itershow,
+                             preallocate=preallocate)
+    if var is not None:
+        var = back(var)
+    return back(x), istop, iiter, r1norm, r2norm, anorm, acond, arnorm, xnorm, var, cost

@@ -1,0 +1,185 @@
+"""GPU tests of the device CGLS / LSQR against the reference's golden solver
+vectors, the oracle's restatement, and library routines
+(pytests/test_solver.py:170-232 restated)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+class _TorchDense:
+    """Dense test operator on the device (torch matmul), float64 or float32."""
+
+    @staticmethod
+    def make(A):
+        import torch
+        from pylops_amd import LinearOperator
+
+        class Dense(LinearOperator):
+            def __init__(self, A):
+                super().__init__(dtype=A.dtype, shape=A.shape)
+                self.At = torch.as_tensor(A, device="cuda")
+
+            # products in float64 (exact reference for the solver), cast back
+            def _matvec(self, x):
+                xd = torch.as_tensor(x, device="cuda").double()
+                return (self.At.double() @ xd).to(self.At.dtype)
+
+            def _rmatvec(self, y):
+                yd = torch.as_tensor(y, device="cuda").double()
+                return (self.At.double().T @ yd).to(self.At.dtype)
+
+        return Dense(A)
+
+
+def _kop(dtype="float64", **kw):
+    import pylops_amd
+    return pylops_amd.Kirchhoff(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                                mode="analytic", dynamic=True, dtype=dtype, **kw)
+
+
+def test_cgls_lsqr_vs_golden():
+    from pylops_amd.optimization.basic import cgls, lsqr
+    g = np.load(os.path.join(G.GOLDEN, "solvers_dyn2d_f64.npz"))
+    Op = _kop()
+    y = g["dobs"]
+    x, istop, iiter, r1, r2, cost = cgls(Op, y, niter=10, tol=0.0)
+    assert (istop, iiter) == (int(g["cgls_istop"]), int(g["cgls_iiter"]))
+    np.testing.assert_allclose(x, g["cgls_x"].reshape(x.shape), rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose([r1, r2], [g["cgls_r1"], g["cgls_r2"]], rtol=1e-7)
+    np.testing.assert_allclose(cost, g["cgls_cost"], rtol=1e-8)
+    x, istop, iiter, r1, r2, cost = cgls(Op, y, x0=g["x0"], niter=8, damp=0.3, tol=0.0)
+    np.testing.assert_allclose(x, g["cglsd_x"], rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose([r1, r2], [g["cglsd_r1"], g["cglsd_r2"]], rtol=1e-7)
+    np.testing.assert_allclose(cost, g["cglsd_cost"], rtol=1e-8)
+    names = ["x", "istop", "iiter", "r1", "r2", "anorm", "acond", "arnorm", "xnorm", "var",
+             "cost"]
+    for pre, kw in (("lsqr_", dict(niter=10)), ("lsqrd_", dict(x0=g["x0"], damp=0.2, niter=7))):
+        res = lsqr(Op, y, calc_var=True, **kw)
+        for k, v in zip(names, res):
+            np.testing.assert_allclose(np.ravel(v), np.ravel(g[pre + k]), rtol=1e-6,
+                                       atol=1e-9, err_msg=pre + k)
+
+
+def test_cgls_tol_early_stop_matches_oracle():
+    from pylops_amd.optimization.basic import cgls
+    Op = _kop()
+    orc = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                            mode="analytic", dynamic=True)
+    g = np.load(os.path.join(G.GOLDEN, "solvers_dyn2d_f64.npz"))
+    y = g["dobs"]
+    # a tolerance between the 4th and the 5th iterate's |r|^2: stops after 5
+    k4 = O.cgls(orc, y, niter=4, tol=0.0)[3]
+    k5 = O.cgls(orc, y, niter=5, tol=0.0)[3]
+    tol = float(np.sqrt(k4 * k5))
+    ref = O.cgls(orc, y, niter=30, tol=tol)
+    res = cgls(Op, y, niter=30, tol=tol)
+    assert (ref[1], ref[2]) == (1, 5)
+    assert res[2] == ref[2] and res[1] == ref[1]
+    np.testing.assert_allclose(res[0].ravel(), ref[0], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(res[5], ref[5], rtol=1e-7)
+
+
+@pytest.mark.parametrize("shape", [(11, 11), (31, 11)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dense_recovery(shape, dtype):
+    """pytests/test_solver.py:170-232: x = ones recovered; LSQR matches
+    scipy.sparse.linalg.lsqr."""
+    from scipy.sparse.linalg import lsqr as sp_lsqr
+
+    from pylops_amd.optimization.basic import cgls, lsqr
+    rng = np.random.default_rng(10)
+    A = (rng.normal(0, 10, shape) + 10 * np.eye(*shape)).astype(dtype)
+    Op = _TorchDense.make(A)
+    xt = np.ones(shape[1], dtype=dtype)
+    y = A @ xt
+    dec = 4 if dtype == np.float64 else 2
+    x = cgls(Op, y, niter=2 * shape[1], tol=0.0)[0]
+    np.testing.assert_array_almost_equal(x, xt, decimal=dec)
+    x0 = rng.standard_normal(shape[1]).astype(dtype)
+    x = cgls(Op, y, x0=x0, niter=2 * shape[1], tol=0.0)[0]
+    np.testing.assert_array_almost_equal(x, xt, decimal=dec)
+    if dtype == np.float64:
+        res = lsqr(Op, y, niter=shape[1], atol=0, btol=0, conlim=0)
+        np.testing.assert_array_almost_equal(res[0], xt, decimal=dec)
+        ref = sp_lsqr(A, y, iter_lim=shape[1], atol=0, btol=0, conlim=0)
+        np.testing.assert_allclose(res[0], ref[0], rtol=1e-6)
+    else:
+        # with float32 operator products n iterations lose orthogonality
+        # (the oracle's LSQR on the same rounded products shows the same
+        # error); compare with it, and check recovery after 2n iterations
+        class Rounded:
+            shape = A.shape
+
+            def matvec(self, v):
+                return (A.astype(np.float64) @ v.astype(np.float32)).astype(np.float32)
+
+            def rmatvec(self, v):
+                return (A.T.astype(np.float64) @ v.astype(np.float32)).astype(np.float32)
+
+        res = lsqr(Op, y, niter=shape[1], atol=0, btol=0, conlim=0)
+        ref = O.lsqr(Rounded(), y.astype(np.float64), niter=shape[1], atol=0, btol=0, conlim=0)
+        np.testing.assert_allclose(res[0], ref[0], rtol=1e-3, atol=1e-4)
+        res = lsqr(Op, y, niter=2 * shape[1], atol=0, btol=0, conlim=0)
+        np.testing.assert_array_almost_equal(res[0], xt, decimal=dec)
+
+
+def test_callbacks_and_show(capsys):
+    from pylops_amd.optimization.basic import cgls, lsqr
+    from pylops_amd.optimization.callback import Callbacks
+    from pylops_amd.optimization.cls_basic import CGLS
+    Op = _kop()
+    g = np.load(os.path.join(G.GOLDEN, "solvers_dyn2d_f64.npz"))
+    seen = []
+    x = cgls(Op, g["dobs"], niter=4, tol=0.0, callback=lambda xx: seen.append(xx.copy()))[0]
+    assert len(seen) == 4 and isinstance(seen[0], np.ndarray)
+    np.testing.assert_allclose(seen[-1].reshape(x.shape), x)
+    lsqr(Op, g["dobs"], niter=3, show=True)
+    out = capsys.readouterr().out
+    assert "LSQR" in out and "istop" in out
+
+    class Count(Callbacks):
+        def __init__(self):
+            self.n = 0
+
+        def on_step_end(self, solver, x):
+            self.n += 1
+
+    cb = Count()
+    import torch
+    s = CGLS(Op, callbacks=[cb])
+    s.solve(torch.as_tensor(g["dobs"], device="cuda"), niter=5, tol=0.0)
+    assert cb.n == 5
+
+
+def test_cgls_float32_config1_vs_float64_oracle():
+    """CGLS of the float32 operator (float64 inner products on the device)
+    against the float64 oracle CGLS on the 2D LSM workload.  Up to 5
+    iterations x agrees to 1e-5.  By iteration 10 the residual is small
+    enough that float32 operator rounding dominates: the oracle's own CGLS on
+    float32-rounded operator products then differs from float64 by 1.8e-3 in
+    x and 1.6e-4 in the residual history, so the bound there is 1e-2 / 1e-3."""
+    from pylops_amd.optimization.basic import cgls
+    from tests.test_kirchhoff_gpu import _config1
+    import pylops_amd
+    kw, _ = _config1()
+    rng = np.random.default_rng(1001)
+    refl = 0.01 * rng.standard_normal((101, 100))
+    for iz in rng.integers(10, 90, 3):
+        refl[:, iz] += rng.uniform(-1, 1)
+    orc = O.KirchhoffOracle(**{**kw, "dtype": "float64"})
+    d = orc.matvec(refl.ravel())
+    Op = pylops_amd.Kirchhoff(**kw)
+    ref5 = O.cgls(orc, d, niter=5, tol=0.0)
+    res5 = cgls(Op, d.astype(np.float32), niter=5, tol=0.0)
+    assert G.relerr(res5[0], ref5[0]) < 1e-5
+    assert G.relerr(res5[5], ref5[5]) < 1e-6
+    ref = O.cgls(orc, d, niter=10, tol=0.0)
+    res = cgls(Op, d.astype(np.float32), niter=10, tol=0.0)
+    assert G.relerr(res[0], ref[0]) < 1e-2
+    assert G.relerr(res[5], ref[5]) < 1e-3
