@@ -214,6 +214,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true",
                     help="also time the 3D workload with float32 and float16 tables")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip timing of the 30-iteration CGLS / LSQR solves")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 4 GiB device copy first (reference kernel for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling)")
@@ -319,11 +321,33 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "model_bytes": int(bytes_alg)},
     }
+    if not args.no_solve:
+        out["solve"] = time_solvers(Op, dobs)
     if args.extra:
         out["extra"] = extra_3d(dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(kw, desc, refl, dobs.cpu().numpy())
     print(json.dumps(out), flush=True)
+
+
+def time_solvers(Op, dobs, niter=30):
+    """Wall time of one CGLS and one LSQR solve of `niter` iterations on the
+    observed data, inputs resident on the device, iterations replayed from
+    a HIP graph (tol = atol = btol = 0, so every iteration runs)."""
+    import torch
+    from pylops_amd.optimization.basic import cgls, lsqr
+    res = {"niter": niter}
+    for name, fn, kw in (("cgls", cgls, dict(tol=0.0)),
+                         ("lsqr", lsqr, dict(atol=0.0, btol=0.0, calc_var=False))):
+        fn(Op, dobs, niter=3, graph=True, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn(Op, dobs, niter=niter, graph=True, **kw)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[f"{name}_s_per_solve"] = round(dt, 4)
+        res[f"{name}_iters"] = int(r[2])
+    return res
 
 
 def extra_3d(dev):

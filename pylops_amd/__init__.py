@@ -15,5 +15,6 @@ from . import waveeqprocessing  # noqa: F401
 from . import optimization  # noqa: F401
 from .signalprocessing import Convolve1D  # noqa: F401
 from .waveeqprocessing import Kirchhoff  # noqa: F401
+from .torchoperator import TorchOperator  # noqa: F401
 
 __version__ = "0.1.0"

@@ -47,28 +47,31 @@ def _prepare(Op, y, x0, callback, solver):
 
 @add_ndarray_support_to_solver
 def cgls(Op, y, x0=None, niter=10, damp=0.0, tol=1e-4, show=False, itershow=(10, 10, 10),
-         callback=None, preallocate=False):
+         callback=None, preallocate=False, graph=False):
     """Conjugate gradient least squares; returns
-    ``(x, istop, iit, r1norm, r2norm, cost)`` as the reference."""
+    ``(x, istop, iit, r1norm, r2norm, cost)`` as the reference.  ``graph``
+    replays a HIP-graph capture of one iteration."""
     cgsolve = CGLS(Op)
     yd, x0d, back = _prepare(Op, y, x0, callback, cgsolve)
     x, istop, iiter, r1norm, r2norm, cost = cgsolve.solve(
         y=yd, x0=x0d, tol=tol, niter=niter, damp=damp, show=show, itershow=itershow,
-        preallocate=preallocate)
+        preallocate=preallocate, graph=graph)
     return back(x), istop, iiter, r1norm, r2norm, cost
 
 
 @add_ndarray_support_to_solver
 def lsqr(Op, y, x0=None, damp=0.0, atol=1e-08, btol=1e-08, conlim=100000000.0, niter=10,
-         calc_var=True, show=False, itershow=(10, 10, 10), callback=None, preallocate=False):
+         calc_var=True, show=False, itershow=(10, 10, 10), callback=None, preallocate=False,
+         graph=False):
     """LSQR; returns ``(x, istop, iiter, r1norm, r2norm, anorm, acond,
-    arnorm, xnorm, var, cost)`` as the reference."""
+    arnorm, xnorm, var, cost)`` as the reference.  ``graph`` replays a
+    HIP-graph capture of one iteration."""
     lsqrsolve = LSQR(Op)
     yd, x0d, back = _prepare(Op, y, x0, callback, lsqrsolve)
     (x, istop, iiter, r1norm, r2norm, anorm, acond, arnorm, xnorm, var,
      cost) = lsqrsolve.solve(y=yd, x0=x0d, damp=damp, atol=atol, btol=btol, conlim=conlim,
                              niter=niter, calc_var=calc_var, show=show, itershow=itershow,
-                             preallocate=preallocate)
+                             preallocate=preallocate, graph=graph)
     if var is not None:
         var = back(var)
     return back(x), istop, iiter, r1norm, r2norm, anorm, acond, arnorm, xnorm, var, cost

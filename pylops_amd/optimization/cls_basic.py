@@ -68,6 +68,35 @@ def _flat(v, device, dtype):
     return v.reshape(-1).to(device=device, dtype=dtype)
 
 
+def _apply_into(Op, x, out, adjoint):
+    """out <- Op x (Op^H x if adjoint).  Operators with a device fast path
+    (the Kirchhoff operator) write straight into ``out``; others go through
+    matvec/rmatvec and a copy."""
+    into = getattr(Op, "_into", None)
+    if into is not None and torch.empty(0, dtype=out.dtype).numpy().dtype == np.dtype(Op.dtype):
+        return into(x, out, adjoint)
+    y = Op.rmatvec(x) if adjoint else Op.matvec(x)
+    out.copy_(_flat(y, out.device, out.dtype))
+    return out
+
+
+def _replay(graph, n, sc, active_index, device):
+    """Replay a captured iteration n times, looking at the loop flag every
+    few replays so that a converged solve stops early."""
+    for k in range(n):
+        if k % _CHECK_EVERY == 0 and k > 0 and sc[active_index].item() == 0.0:
+            break
+        graph.replay()
+
+
+def _capture(body, device):
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize(device)
+    with torch.cuda.graph(graph):
+        body()
+    return graph
+
+
 class CGLS(Solver):
     r"""Conjugate gradient least squares: minimises
     :math:`\|\mathbf{y} - \mathbf{Op}\,\mathbf{x}\|_2^2 + \epsilon^2\|\mathbf{x}\|_2^2`
@@ -129,7 +158,9 @@ class CGLS(Solver):
             x = _flat(x0, dev, dt).clone()
             self.s = y - _flat(Op.matvec(x), dev, dt)
             self.c = _flat(Op.rmatvec(self.s), dev, dt) - damp * x  # raw damp (CB:477)
-        self.q = _flat(Op.matvec(self.c), dev, dt)
+        self.q = torch.empty(Op.shape[0], dtype=dt, device=dev)
+        self.r = torch.empty(Op.shape[1], dtype=dt, device=dev)
+        _apply_into(Op, self.c, self.q, False)
         D.dot(self.c, self.c, _lib.KX_CG_KOLD)
         D.dot(self.c, self.c, _lib.KX_CG_CC)
         D.dot(self.q, self.q, _lib.KX_CG_QQ)
@@ -150,27 +181,32 @@ class CGLS(Solver):
             self._print_setup()
         return x
 
-    def step(self, x, show=False):
-        """One CGLS iteration (CB:510-564), enqueued on the device."""
+    def _step_body(self, x):
+        """Enqueue one CGLS iteration (CB:510-564); touches only persistent
+        buffers, so it can be captured in a graph and replayed."""
         D = self._d
         L = D.L
         A = _lib.KX_CG_ACTIVE
-        self._grow(self.iiter + 2)
         st = D.stream()
         cost, cost1 = _lib.ptr(self._cost), _lib.ptr(self._cost1)
         _lib.check(L.kx_cgls_scalar(D.s(0), cost, cost1, 0, st), "kx_cgls_scalar")
         D.vec(_lib.KX_VEC_AXPY, self.c, x, _lib.KX_CG_A, 1.0, _lib.KX_CG_XX, A)
         D.vec(_lib.KX_VEC_AXPY, self.q, self.s, _lib.KX_CG_A, -1.0, _lib.KX_CG_SS, A)
-        r = _flat(self.Op.rmatvec(self.s), D.device, D.dtype)
+        _apply_into(self.Op, self.s, self.r, True)
         if self.damp != 0:
-            D.vec(_lib.KX_VEC_AXPY, x, r, _lib.KX_CG_DAMP2, -1.0, _lib.KX_CG_K, A)
+            D.vec(_lib.KX_VEC_AXPY, x, self.r, _lib.KX_CG_DAMP2, -1.0, _lib.KX_CG_K, A)
         else:
-            D.dot(r, r, _lib.KX_CG_K, A)
+            D.dot(self.r, self.r, _lib.KX_CG_K, A)
         _lib.check(L.kx_cgls_scalar(D.s(0), cost, cost1, 1, st), "kx_cgls_scalar")
-        D.vec(_lib.KX_VEC_XPBY, r, self.c, _lib.KX_CG_B, 1.0, _lib.KX_CG_CC, A)
-        self.q = _flat(self.Op.matvec(self.c), D.device, D.dtype)
+        D.vec(_lib.KX_VEC_XPBY, self.r, self.c, _lib.KX_CG_B, 1.0, _lib.KX_CG_CC, A)
+        _apply_into(self.Op, self.c, self.q, False)
         D.dot(self.q, self.q, _lib.KX_CG_QQ, A)
         _lib.check(L.kx_cgls_scalar(D.s(0), cost, cost1, 2, st), "kx_cgls_scalar")
+
+    def step(self, x, show=False):
+        """One CGLS iteration (CB:510-564), enqueued on the device."""
+        self._grow(self.iiter + 2)
+        self._step_body(x)
         self.iiter += 1
         if show:
             self._print_step(x)
@@ -181,11 +217,13 @@ class CGLS(Solver):
         self.iiter = int(h[_lib.KX_CG_ITER])
         return h
 
-    def run(self, x, niter=None, show=False, itershow=(10, 10, 10)):
-        """CB:566-611."""
+    def run(self, x, niter=None, show=False, itershow=(10, 10, 10), graph=False):
+        """CB:566-611.  ``graph=True`` runs the first iteration eagerly,
+        captures the next one in a HIP graph and replays it."""
         niter = self.niter if niter is None else niter
         if niter is None:
             raise ValueError("niter must not be None")
+        self._grow(niter + 1)
         self._d.sc[_lib.KX_CG_NITER] = float(niter)
         self._d.sc[_lib.KX_CG_ACTIVE] = (
             (self._d.sc[_lib.KX_CG_ITER] < niter)
@@ -199,6 +237,12 @@ class CGLS(Solver):
                                      or self.iiter % itershow[2] == 0)
                 x = self.step(x, showstep)
                 self.callback(x)
+        elif graph and niter - self.iiter > 1:
+            x = self.step(x)
+            g = _capture(lambda: self._step_body(x), self._d.device)
+            _replay(g, niter - self.iiter, self._d.sc, _lib.KX_CG_ACTIVE, self._d.device)
+            self._graph = g
+            self._sync_state()
         else:
             start = self.iiter
             for k in range(max(niter - start, 0)):
@@ -227,10 +271,10 @@ class CGLS(Solver):
             self._print_finalize(nbar=65)
 
     def solve(self, y, x0=None, niter=10, damp=0.0, tol=1e-4, preallocate=False, show=False,
-              itershow=(10, 10, 10)):
+              itershow=(10, 10, 10), graph=False):
         x = self.setup(y=y, x0=x0, niter=niter, damp=damp, tol=tol, preallocate=preallocate,
                        show=show)
-        x = self.run(x, niter, show=show, itershow=itershow)
+        x = self.run(x, niter, show=show, itershow=itershow, graph=graph)
         self.finalize(show)
         return x, self.istop, self.iiter, self.r1norm, self.r2norm, self.cost
 
@@ -338,6 +382,8 @@ class LSQR(Solver):
             self.v = x.clone()
             alfa = 0.0
         self.w = self.v.clone()
+        self.tmp_d = torch.empty(m, dtype=dt, device=dev)
+        self.tmp_m = torch.empty(n, dtype=dt, device=dev)
         self.alfa, self.beta = alfa, beta
         arnorm = alfa * beta
         self.arnorm = arnorm
@@ -374,21 +420,20 @@ class LSQR(Solver):
             print(self.msg[self.istop])
         return x
 
-    def step(self, x, show=False):
-        """One LSQR iteration (CB:987-1136), enqueued on the device."""
+    def _step_body(self, x):
+        """Enqueue one LSQR iteration (CB:987-1136) on persistent buffers."""
         D = self._d
         L = D.L
         A = _lib.KX_LS_ACTIVE
-        self._grow(self.iiter + 2)
         st = D.stream()
         cost = _lib.ptr(self._cost)
-        tmp = _flat(self.Op.matvec(self.v), D.device, D.dtype)
-        D.vec(_lib.KX_VEC_XPBY, tmp, self.u, _lib.KX_LS_ALFA, -1.0, _lib.KX_LS_UU, A)
+        _apply_into(self.Op, self.v, self.tmp_d, False)
+        D.vec(_lib.KX_VEC_XPBY, self.tmp_d, self.u, _lib.KX_LS_ALFA, -1.0, _lib.KX_LS_UU, A)
         _lib.check(L.kx_lsqr_scalar(D.s(0), cost, 0, st), "kx_lsqr_scalar")
         D.vec(_lib.KX_VEC_SCALE, None, self.u, _lib.KX_LS_BETA, 1.0, None, A,
               _lib.KX_LS_BETAPOS)
-        tmp2 = _flat(self.Op.rmatvec(self.u), D.device, D.dtype)
-        D.vec(_lib.KX_VEC_XPBY, tmp2, self.v, _lib.KX_LS_BETA, -1.0, _lib.KX_LS_VV, A,
+        _apply_into(self.Op, self.u, self.tmp_m, True)
+        D.vec(_lib.KX_VEC_XPBY, self.tmp_m, self.v, _lib.KX_LS_BETA, -1.0, _lib.KX_LS_VV, A,
               _lib.KX_LS_BETAPOS)
         _lib.check(L.kx_lsqr_scalar(D.s(0), cost, 1, st), "kx_lsqr_scalar")
         D.vec(_lib.KX_VEC_SCALE, None, self.v, _lib.KX_LS_ALFA, 1.0, None, A,
@@ -398,6 +443,11 @@ class LSQR(Solver):
                           D.s(0), _lib.ptr(D.work), st)
         _lib.check(rc, "kx_lsqr_xw")
         _lib.check(L.kx_lsqr_scalar(D.s(0), cost, 3, st), "kx_lsqr_scalar")
+
+    def step(self, x, show=False):
+        """One LSQR iteration (CB:987-1136), enqueued on the device."""
+        self._grow(self.iiter + 2)
+        self._step_body(x)
         self.iiter += 1
         if show:
             self._print_step(x)
@@ -409,9 +459,11 @@ class LSQR(Solver):
         self.istop = int(h[_lib.KX_LS_ISTOP])
         return h
 
-    def run(self, x, niter=None, show=False, itershow=(10, 10, 10)):
-        """CB:1138-1186."""
+    def run(self, x, niter=None, show=False, itershow=(10, 10, 10), graph=False):
+        """CB:1138-1186.  ``graph=True``: first iteration eager, the next one
+        captured in a HIP graph and replayed."""
         niter = self.niter if niter is None else niter
+        self._grow(niter + 1)
         D = self._d
         D.sc[_lib.KX_LS_NITER] = float(niter)
         D.sc[_lib.KX_LS_ACTIVE] = ((D.sc[_lib.KX_LS_ITER] < niter)
@@ -429,6 +481,12 @@ class LSQR(Solver):
                     or h[_lib.KX_LS_TEST1] <= 10 * h[_lib.KX_LS_RTOL])
                 x = self.step(x, showstep)
                 self.callback(x)
+        elif graph and niter - self.iiter > 1:
+            x = self.step(x)
+            g = _capture(lambda: self._step_body(x), D.device)
+            _replay(g, niter - self.iiter, D.sc, _lib.KX_LS_ACTIVE, D.device)
+            self._graph = g
+            self._sync_state()
         else:
             start = self.iiter
             for k in range(max(niter - start, 0)):
@@ -459,10 +517,11 @@ class LSQR(Solver):
             self._print_finalize()
 
     def solve(self, y, x0=None, damp=0.0, atol=1e-08, btol=1e-08, conlim=100000000.0,
-              niter=10, calc_var=True, preallocate=False, show=False, itershow=(10, 10, 10)):
+              niter=10, calc_var=True, preallocate=False, show=False, itershow=(10, 10, 10),
+              graph=False):
         x = self.setup(y=y, x0=x0, damp=damp, atol=atol, btol=btol, conlim=conlim,
                        niter=niter, calc_var=calc_var, preallocate=preallocate, show=show)
-        x = self.run(x, niter=niter, show=show, itershow=itershow)
+        x = self.run(x, niter=niter, show=show, itershow=itershow, graph=graph)
         self.finalize(show)
         return (x, self.istop, self.iiter, self.r1norm, self.r2norm, self.anorm, self.acond,
                 self.arnorm, self.xnorm, self.var, self.cost)

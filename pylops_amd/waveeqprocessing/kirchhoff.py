@@ -362,6 +362,27 @@ class Kirchhoff(LinearOperator):
     def _work(self, device, tdtype):
         return torch.empty(self.nsnr * self.nt, dtype=tdtype, device=device)
 
+    def _into(self, x, out, adjoint):
+        """Device fast path for the solvers: apply the operator (adjoint if
+        ``adjoint``) to the contiguous device tensor ``x`` and write the
+        result into ``out``, reusing one cached workspace, so that a captured
+        iteration always touches the same buffers."""
+        plan = self.plan
+        ws = getattr(self, "_ws", None)
+        if ws is None or ws.dtype != x.dtype or ws.device != x.device:
+            ws = torch.empty(self.nsnr * self.nt, dtype=x.dtype, device=x.device)
+            object.__setattr__(self, "_ws", ws)
+        L = _lib.load()
+        fn = L.kx_adjoint if adjoint else L.kx_forward
+        rc = fn(ctypes.byref(plan), _lib.ptr(x), _lib.ptr(out), _lib.ptr(ws),
+                _lib.stream_ptr(x.device))
+        _lib.check(rc, "kx_adjoint" if adjoint else "kx_forward")
+        if adjoint:
+            self.rmatvec_count += 1
+        else:
+            self.matvec_count += 1
+        return out
+
     def _launch(self, fn, x, nout, what):
         plan = self.plan
         xd = to_device(x.reshape(-1), self.dtype, self.device)

@@ -1,0 +1,89 @@
+"""TorchOperator / autograd on the HIP operator
+(pytests/test_torchoperator.py:28-103 restated), plus a central
+finite-difference check of the gradient of 0.5 |K m - d|^2."""
+import numpy as np
+import pytest
+
+from tests import geometry as G
+from tests.test_solvers_gpu import _TorchDense
+
+pytestmark = pytest.mark.gpu
+
+
+def _kop(dtype):
+    import pylops_amd
+    return pylops_amd.Kirchhoff(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                                mode="analytic", dynamic=True, dtype=dtype)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_torchoperator_forward_and_gradient(dtype):
+    import torch
+    from pylops_amd import TorchOperator
+    Op = _kop(dtype)
+    Top = TorchOperator(Op, batch=False, device="gpu")
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(Op.shape[1]).astype(dtype)
+    v = rng.standard_normal(Op.shape[0]).astype(dtype)
+    xt = torch.as_tensor(x, device="cuda").view(-1)
+    xt.requires_grad = True
+    vt = torch.as_tensor(v, device="cuda").view(-1)
+    yt = Top.apply(xt)
+    yt.backward(vt, retain_graph=True)
+    tol = 1e-6 if dtype == "float32" else 1e-14
+    # forward / gradient are the operator's own kernels (LDS float64
+    # accumulation order may change the last float32 bit between calls)
+    np.testing.assert_allclose(yt.detach().cpu().numpy(), Op @ x, rtol=tol,
+                               atol=tol * np.abs(Op @ x).max())
+    np.testing.assert_allclose(xt.grad.cpu().numpy(), Op.H @ v, rtol=tol,
+                               atol=tol * np.abs(Op.H @ v).max())
+
+
+def test_torchoperator_batch_dense():
+    import torch
+    from pylops_amd import TorchOperator
+    rng = np.random.default_rng(1)
+    A = rng.normal(0.0, 1.0, (11, 11)).astype(np.float32)
+    Dop = _TorchDense.make(A)
+    Top = TorchOperator(Dop, batch=True, device="gpu")
+    x = rng.normal(0.0, 1.0, (4, 11)).astype(np.float32)
+    xt = torch.as_tensor(x, device="cuda")
+    xt.requires_grad = True
+    yt = Top.apply(xt)
+    np.testing.assert_allclose(yt.detach().cpu().numpy(), (A.astype(np.float64) @ x.T).T,
+                               rtol=1e-6, atol=1e-6)
+    yt.sum().backward()
+    np.testing.assert_allclose(xt.grad.cpu().numpy(),
+                               np.tile(A.astype(np.float64).sum(0), (4, 1)), rtol=1e-5)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_gradient_finite_differences(dtype):
+    """loss = 0.5 |K m - d|^2: the gradient from the adjoint kernel matches
+    central differences (eps = 1e-3) along 32 random unit directions.  The
+    loss is quadratic, so central differences are exact up to rounding."""
+    import torch
+    from pylops_amd import TorchOperator
+    Op = _kop(dtype)
+    Top = TorchOperator(Op)
+    tdt = torch.float64 if dtype == "float64" else torch.float32
+    rng = np.random.default_rng(2)
+    d = torch.as_tensor(rng.standard_normal(Op.shape[0]), dtype=tdt, device="cuda")
+    m0 = torch.as_tensor(rng.standard_normal(Op.shape[1]), dtype=tdt, device="cuda")
+    m = m0.clone().requires_grad_(True)
+    loss = 0.5 * ((Top.apply(m) - d) ** 2).sum()
+    loss.backward()
+    g = m.grad.double()
+    eps = 1e-3
+    errs = []
+    for k in range(32):
+        u = torch.as_tensor(rng.standard_normal(Op.shape[1]), dtype=torch.float64, device="cuda")
+        u /= u.norm()
+        with torch.no_grad():
+            lp = 0.5 * ((Top.apply((m0.double() + eps * u).to(tdt)).double() - d.double()) ** 2).sum()
+            lm = 0.5 * ((Top.apply((m0.double() - eps * u).to(tdt)).double() - d.double()) ** 2).sum()
+        fd = float((lp - lm) / (2 * eps))
+        an = float((g * u).sum())
+        errs.append(abs(fd - an) / max(abs(an), 1e-12 * float(g.norm())))
+    tol = 1e-6 if dtype == "float64" else 2e-2
+    assert np.median(errs) < tol, errs
