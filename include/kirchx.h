@@ -39,7 +39,17 @@
 extern "C" {
 #endif
 
-#define KX_ABI_VERSION 1
+#define KX_ABI_VERSION 2
+
+/* Bytes of the per-plan device scratch `aux` (forward accumulation scale). */
+#define KX_AUX_BYTES 4096
+
+/* lane order of the forward (kx_plan.sorder): at step r of a tile the lane
+ * of point p treats receiver r (natural) or (p + r) mod rblock (rotated) of
+ * its workgroup's block */
+#define KX_ORDER_AUTO 0
+#define KX_ORDER_NATURAL 1
+#define KX_ORDER_ROTATED 2
 
 /* element types */
 #define KX_F32 0
@@ -56,10 +66,11 @@ extern "C" {
 
 /* Operator plan.  All table pointers are device pointers in the tiled
  * layout produced by kx_encode_table / kx_tile_table:
- *   offsets  off[n][ntiles][64]   element type tab_fmt (f32, f16 or f64),
+ *   offsets  off[ntiles][n][64]   element type tab_fmt (f32, f16 or f64),
  *            traveltime in samples minus base,
- *   bases    base[n][ntiles]      int32, samples,
- *   weights  wp/wq[n][ntiles][64] element type dtype (dynamic only).
+ *   bases    base[ntiles][n]      int32, samples,
+ *   weights  wp/wq[ntiles][n][64] element type dtype (dynamic only),
+ * n = ns for the source side and nr for the receiver side.
  * A tile is a tz*tx*ty = 64 block of image points; tiles are enumerated
  * z fastest, then x, then y; lanes inside a tile likewise. */
 typedef struct kx_plan {
@@ -75,7 +86,7 @@ typedef struct kx_plan {
     int32_t neg_trav;    /* 1 if a traveltime may be negative (truncation path) */
     int32_t rblock;      /* receivers per spread workgroup, 0 = automatic */
     int32_t nh;          /* wavelet length after offset padding (odd) */
-    int32_t pad0;
+    int32_t sorder;      /* spread lane order KX_ORDER_*, 0 = automatic */
     const void *off_s;
     const int32_t *base_s;
     const void *off_r;
@@ -84,6 +95,11 @@ typedef struct kx_plan {
     const int32_t *izmin;                  /* [ns*nr] aperture gate or NULL */
     const void *h;                         /* wavelet, nh elements of dtype */
     const void *hstar;                     /* flipped wavelet */
+    double wamp;  /* bound on |weight| of one (point, trace) pair:
+                     1 kinematic, max_s |(wp_s, wq_s)| * max_r |(wp_r, wq_r)| dynamic */
+    void *aux;    /* KX_AUX_BYTES of device memory owned by the caller; the
+                     forward writes its accumulation scale there, so calls that
+                     share one aux must be ordered on one stream */
 } kx_plan;
 
 int kx_version(void);
@@ -101,7 +117,7 @@ int kx_encode_table(const kx_plan *geom, const double *tabT, int32_t n,
                     double scale, int32_t fmt, void *off, int32_t *base,
                     void *stream);
 
-/* Reorders a [n][ni] float64 table into [n][ntiles][64] of dtype `fmt`
+/* Reorders a [n][ni] float64 table into [ntiles][n][64] of dtype `fmt`
  * (KX_F32 or KX_F64), zero padded (dynamic weights). */
 int kx_tile_table(const kx_plan *geom, const double *tabT, int32_t n,
                   int32_t fmt, void *out, void *stream);

@@ -12,8 +12,10 @@ import torch  # noqa: F401  (must precede the HIP library)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libkirchx.so")
 
-KX_ABI_VERSION = 1
+KX_ABI_VERSION = 2
+KX_AUX_BYTES = 4096
 KX_F32, KX_F64, KX_F16 = 0, 1, 2
+KX_ORDER_AUTO, KX_ORDER_NATURAL, KX_ORDER_ROTATED = 0, 1, 2
 KX_EINVAL, KX_ESHAPE, KX_ETYPE, KX_EABI, KX_ELDS = -1, -2, -3, -4, -5
 
 EXPORTS = (
@@ -106,7 +108,7 @@ class KxPlan(ctypes.Structure):
         ("neg_trav", ctypes.c_int32),
         ("rblock", ctypes.c_int32),
         ("nh", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("sorder", ctypes.c_int32),
         ("off_s", ctypes.c_void_p),
         ("base_s", ctypes.c_void_p),
         ("off_r", ctypes.c_void_p),
@@ -116,6 +118,8 @@ class KxPlan(ctypes.Structure):
         ("izmin", ctypes.c_void_p),
         ("h", ctypes.c_void_p),
         ("hstar", ctypes.c_void_p),
+        ("wamp", ctypes.c_double),
+        ("aux", ctypes.c_void_p),
     ]
 
 

@@ -4,8 +4,8 @@
 // Data layout in HBM (all caller-owned):
 //   model  m[ni]              reference order ii = (iy*nx + ix)*nz + iz
 //   data   d[ns*nr][nt]       trace isrc*nr + irec
-//   tables off[n][ntiles][64] traveltime in samples minus a per-(n, tile)
-//                             integer base[n][ntiles]; a tile is a
+//   tables off[ntiles][n][64] traveltime in samples minus a per-(tile, n)
+//                             integer base[ntiles][n]; a tile is a
 //                             tz*tx*ty = 64 block of image points, one lane
 //                             of a wavefront per point
 // The split into an integer base and a small offset keeps the fractional
@@ -15,8 +15,8 @@
 // Forward (spread, replaces K:727-752 / K:783-889): one workgroup owns one
 // source and a block of receivers; their traces live in LDS, image tiles are
 // streamed through the waves, every (point, trace) pair adds two linearly
-// interpolated samples with LDS float atomics, and the traces are written to
-// HBM once.  No global atomics.
+// interpolated samples with LDS atomics (64-bit fixed point for float data),
+// and the traces are written to HBM once.  No global atomics.
 // Adjoint (gather, replaces K:754-781 / K:891-1001): one wavefront owns one
 // image tile; source offsets for a block of sources are held in registers,
 // receiver offsets are streamed coalesced, and each lane gathers its two
@@ -82,11 +82,6 @@ constexpr int kReduceBlocks = 1024;
 template <typename TD> struct TimeType { using type = float; };
 template <> struct TimeType<double> { using type = double; };
 
-// LDS trace accumulator of the forward.  float64: on gfx950 an LDS float64
-// atomic add sustains ~3 lanes/clk/CU against ~0.3 for float32, and the
-// traces come out more accurate.
-using AccT = double;
-
 __device__ __forceinline__ float ld_off(const float *p) { return *p; }
 __device__ __forceinline__ float ld_off(const __half *p) { return __half2float(*p); }
 __device__ __forceinline__ double ld_off(const double *p) { return *p; }
@@ -147,17 +142,51 @@ __device__ __forceinline__ int64_t point_index(const KArgs &a, const TilePoint &
 // ---------------------------------------------------------------------------
 // forward: spread into LDS-resident traces
 // ---------------------------------------------------------------------------
-// Everything one tile of one wave needs: the image value and the source /
-// receiver offsets, bases and weights of its 64 points.
-template <typename TD, typename TT, int RB> struct SpreadTile {
-    TD mv;
-    TT os;  // raw table elements: converted at use, so a prefetch never waits
-    int bs;
-    TT orr[RB];
-    int br[RB];
-    TD ps, qs;
-    TD pr[RB], qr[RB];
-    int iz;
+// LDS accumulation.  For float data every contribution c (rounded to float
+// as the reference rounds it) is added as the integer rint(c * 2^e) to a
+// 64-bit fixed-point sample, with one power-of-two scale 2^e per launch
+// chosen from max|m| so that |c| 2^e <= 2^30: the rounding of a contribution
+// is at most 2^-31 of the largest possible one, the sum itself is exact and
+// does not depend on the order of the adds, and it cannot overflow (at most
+// 2^32 contributions per sample).  On gfx950 LDS integer atomics sustain
+// ~1.5x the rate of float64 ones and float32 LDS atomics ~1/6 of float64.
+// A launch whose m holds inf or NaN accumulates in float64 instead, so that
+// they propagate as in the reference; double data always does.
+constexpr int kSpreadWaves = 8;
+constexpr int kSpreadBlock = kSpreadWaves * kWave;
+constexpr int kScaleBlocks = 256;
+
+// aux layout: [0] int exponent e, [1] int 1 = fixed point / 0 = float64,
+// [16, 16 + kScaleBlocks) float partial maxima
+struct SpreadScale {
+    int e;
+    int fixed;
+};
+
+// Running tile coordinates of a wave: tiles are visited t, t + step, ...;
+// advancing carries through z, then x, then y without a division.
+struct TileCursor {
+    int t, tzi, txi, tyi;
+    __device__ __forceinline__ void start(const KArgs &a, int t0)
+    {
+        t = t0;
+        tzi = t0 % a.ntz;
+        const int rest = t0 / a.ntz;
+        txi = rest % a.ntx;
+        tyi = rest / a.ntx;
+    }
+    __device__ __forceinline__ void advance(const KArgs &a, int step)
+    {
+        t += step;
+        tzi += step;
+        while (tzi >= a.ntz) {
+            tzi -= a.ntz;
+            if (++txi == a.ntx) {
+                txi = 0;
+                ++tyi;
+            }
+        }
+    }
 };
 
 // Opaque zero held in a VGPR: indexing with it keeps a wave-uniform load on
@@ -171,44 +200,72 @@ __device__ __forceinline__ int vgpr_zero()
     return z;
 }
 
-template <typename TD, typename TT, bool DYN, int RB>
-__device__ __forceinline__ void spread_load(const KArgs &a, int t, int lane, int nrl, int vz,
-                                            const TD *__restrict__ m, const TT *offs,
-                                            const int *bss, const TT *offr, const int *bsr,
-                                            const TD *wps, const TD *wqs, const TD *wpr,
-                                            const TD *wqr, size_t rstride,
+// Lane roles.  Lane p holds point p of the tile and reads the source side
+// of its point once per tile.  At step r (r < RB) it treats receiver j of
+// the workgroup's block: j = r in natural order, so that the table loads of
+// a step are coalesced rows, or j = (p + r) mod RB in rotated order, so that
+// one atomic instruction covers all RB trace rows at 64 / RB points RB apart
+// each.  Samples of neighbouring points of one trace often coincide, and the
+// lanes of an LDS atomic that hit one address are serialised: the rotated
+// order trades scattered table loads for fewer such collisions.
+template <int RB, bool ROT> __device__ __forceinline__ int step_receiver(int lane, int r)
+{
+    return ROT ? (lane + r) & (RB - 1) : r;
+}
+
+// One lane's share of one tile: the image value and the table entries of
+// its point, receivers in the lane's step order.
+template <typename TD, typename TT, int RB> struct SpreadTile {
+    TD mv;
+    TT os;       // raw table elements: converted at use, so that a prefetch
+    TT orr[RB];  // never waits for them
+    TD ps, qs;
+    TD pr[RB], qr[RB];
+    int bs;
+    int br[RB];
+};
+
+template <typename TD, typename TT, bool DYN, bool ROT, int RB>
+__device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c, int lane, int rel,
+                                            int lz, int lx, int ly, int s, int r0, int nrl,
+                                            const TD *__restrict__ m, int vz,
                                             SpreadTile<TD, TT, RB> &L)
 {
-    const TilePoint p = tile_point(a, t, lane);
-    L.iz = p.iz;
-    L.mv = p.valid ? m[point_index(a, p)] : TD(0);
-    const size_t tl = (size_t)t * kWave + lane;
-    L.os = offs[tl];
-    L.bs = bss[t + vz];
+    const int t = c.t;
+    const int iz = c.tzi * a.tz + lz, ix = c.txi * a.tx + lx, iy = c.tyi * a.ty + ly;
+    const bool valid = (iz < a.nz) & (ix < a.nx) & (iy < a.ny);
+    const int orig = (c.tyi * a.ty * a.nx + c.txi * a.tx) * a.nz + c.tzi * a.tz;
+    L.mv = valid ? m[orig + rel] : TD(0);
+    const size_t ts = (size_t)t * a.ns + s;        // (tile, source) row
+    const size_t tr = (size_t)t * a.nr + r0;       // (tile, first receiver) row
+    L.os = reinterpret_cast<const TT *>(a.off_s)[ts * kWave + lane];
+    L.bs = a.base_s[ts + vz];
     if constexpr (DYN) {
-        L.ps = wps[tl];
-        L.qs = wqs[tl];
+        L.ps = reinterpret_cast<const TD *>(a.wp_s)[ts * kWave + lane];
+        L.qs = reinterpret_cast<const TD *>(a.wq_s)[ts * kWave + lane];
     }
 #pragma unroll
-    for (int j = 0; j < RB; ++j) {
-        if (j < nrl) {
-            L.orr[j] = offr[j * rstride + tl];
-            L.br[j] = bsr[(size_t)j * a.ntiles + t + vz];
-            if constexpr (DYN) {
-                L.pr[j] = wpr[j * rstride + tl];
-                L.qr[j] = wqr[j * rstride + tl];
-            }
+    for (int r = 0; r < RB; ++r) {
+        int j = step_receiver<RB, ROT>(lane, r);
+        j = j < nrl ? j : 0;  // rows past the last receiver read row 0
+        const size_t e = (tr + j) * kWave + lane;
+        L.orr[r] = reinterpret_cast<const TT *>(a.off_r)[e];
+        L.br[r] = a.base_r[tr + j];
+        if constexpr (DYN) {
+            L.pr[r] = reinterpret_cast<const TD *>(a.wp_r)[e];
+            L.qr[r] = reinterpret_cast<const TD *>(a.wq_r)[e];
         }
     }
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int RB>
-__global__ __launch_bounds__(kBlock) void spread_kernel(KArgs a, const TD *__restrict__ m,
-                                                        TD *__restrict__ y)
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
+__device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
+                                            TD *__restrict__ y, int e)
 {
     using TC = typename TimeType<TD>::type;
     extern __shared__ __align__(16) unsigned char kx_smem[];
-    AccT *acc = reinterpret_cast<AccT *>(kx_smem);
+    double *accd = reinterpret_cast<double *>(kx_smem);
+    unsigned long long *acci = reinterpret_cast<unsigned long long *>(kx_smem);
 
     // one workgroup = one source s and receivers [r0, r0 + RB)
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -218,50 +275,50 @@ __global__ __launch_bounds__(kBlock) void spread_kernel(KArgs a, const TD *__res
     const int ntr = nrl * a.nt;
 
     for (int i = threadIdx.x; i < ntr; i += blockDim.x)
-        acc[i] = AccT(0);
+        acci[i] = 0ull;  // 0.0 and integer 0 share the bit pattern
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int nw = blockDim.x / kWave;
     const int vz = vgpr_zero();
-    const size_t rstride = (size_t)a.ntiles * kWave;  // next receiver row
-    const TT *offs = reinterpret_cast<const TT *>(a.off_s) + (size_t)s * rstride;
-    const TT *offr = reinterpret_cast<const TT *>(a.off_r) + (size_t)r0 * rstride;
-    const int *bss = a.base_s + (size_t)s * a.ntiles;
-    const int *bsr = a.base_r + (size_t)r0 * a.ntiles;
-    const TD *wps = reinterpret_cast<const TD *>(a.wp_s) + (size_t)s * rstride;
-    const TD *wqs = reinterpret_cast<const TD *>(a.wq_s) + (size_t)s * rstride;
-    const TD *wpr = reinterpret_cast<const TD *>(a.wp_r) + (size_t)r0 * rstride;
-    const TD *wqr = reinterpret_cast<const TD *>(a.wq_r) + (size_t)r0 * rstride;
-
-    int izmin[RB];
-    if constexpr (DYN) {
+    const int lz = lane % a.tz, lx = (lane / a.tz) % a.tx, ly = lane / (a.tz * a.tx);
+    const int rel = (ly * a.nx + lx) * a.nz + lz;  // image offset from the tile origin
+    int lrow[RB];                                  // LDS row of the receiver of step r
+    int izm[RB];
 #pragma unroll
-        for (int j = 0; j < RB; ++j)
-            izmin[j] = j < nrl ? a.izmin[s * a.nr + r0 + j] : 0;
+    for (int r = 0; r < RB; ++r) {
+        const int j = step_receiver<RB, ROT>(lane, r);
+        lrow[r] = j < nrl ? j * a.nt : -1;
+        if constexpr (DYN)
+            izm[r] = a.izmin[s * a.nr + r0 + (j < nrl ? j : 0)];
     }
+    const unsigned ntm1 = (unsigned)(a.nt - 1);
 
     SpreadTile<TD, TT, RB> cur, nxt;
-    int t = wave;
-    if (t < a.ntiles)
-        spread_load<TD, TT, DYN, RB>(a, t, lane, nrl, vz, m, offs, bss, offr, bsr, wps, wqs,
-                                     wpr, wqr, rstride, cur);
-    for (; t < a.ntiles; t += nw) {
+    TileCursor c;
+    c.start(a, wave);
+    if (c.t < a.ntiles)
+        spread_load<TD, TT, DYN, ROT, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, cur);
+    while (c.t < a.ntiles) {
         // prefetch the next tile while this one is accumulated
-        if (t + nw < a.ntiles)
-            spread_load<TD, TT, DYN, RB>(a, t + nw, lane, nrl, vz, m, offs, bss, offr, bsr,
-                                         wps, wqs, wpr, wqr, rstride, nxt);
+        TileCursor cn = c;
+        cn.advance(a, nw);
+        if (cn.t < a.ntiles)
+            spread_load<TD, TT, DYN, ROT, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
         if (__ballot(cur.mv != TD(0)) != 0) {
-            int it[RB];
-            AccT w0[RB], w1[RB];
+            const int iz = c.tzi * a.tz + lz;
             const TC os = (TC)ld_off(&cur.os);
+            // the image value, scaled by 2^e for the fixed-point sum
+            TD mv = cur.mv;
+            if constexpr (FIX)
+                mv = ldexpf(mv, e);
 #pragma unroll
-            for (int j = 0; j < RB; ++j) {
-                const TC T = os + (TC)ld_off(&cur.orr[j]);
+            for (int r = 0; r < RB; ++r) {
+                const TC T = os + (TC)ld_off(&cur.orr[r]);
                 const TC fl = floor(T);
                 TC f = T - fl;
-                int k = (int)fl + cur.bs + cur.br[j];
+                int k = (int)fl + cur.bs + cur.br[r];
                 if constexpr (NEG) {
                     // the reference truncates toward zero: a time in (-1, 0)
                     // lands on sample 0 with a negative weight
@@ -270,30 +327,111 @@ __global__ __launch_bounds__(kBlock) void spread_kernel(KArgs a, const TD *__res
                         f -= TC(1);
                     }
                 }
-                TD w = cur.mv;
+                TD w = mv;
                 if constexpr (DYN) {
-                    w = cur.mv * (cur.ps * cur.pr[j] + cur.qs * cur.qr[j]);
-                    if (cur.iz < izmin[j])
+                    w = w * (cur.ps * cur.pr[r] + cur.qs * cur.qr[r]);
+                    if (iz < izm[r])
                         w = TD(0);
                 }
-                it[j] = (j < nrl && (unsigned)k < (unsigned)(a.nt - 1)) ? j * a.nt + k : -1;
-                w0[j] = (AccT)(w * (TD)(TC(1) - f));
-                w1[j] = (AccT)(w * (TD)f);
-            }
-#pragma unroll
-            for (int j = 0; j < RB; ++j) {
-                if (it[j] >= 0) {
-                    atomicAdd(&acc[it[j]], w0[j]);
-                    atomicAdd(&acc[it[j] + 1], w1[j]);
+                if (lrow[r] >= 0 && (unsigned)k < ntm1 && w != TD(0)) {
+                    const int it = lrow[r] + k;
+                    const TD w0 = w * (TD)(TC(1) - f);
+                    const TD w1 = w * (TD)f;
+                    if constexpr (FIX) {
+                        atomicAdd(&acci[it], (unsigned long long)(long long)__float2int_rn(w0));
+                        atomicAdd(&acci[it + 1], (unsigned long long)(long long)__float2int_rn(w1));
+                    } else {
+                        atomicAdd(&accd[it], (double)w0);
+                        atomicAdd(&accd[it + 1], (double)w1);
+                    }
                 }
             }
         }
         cur = nxt;
+        c = cn;
     }
     __syncthreads();
     TD *yo = y + ((size_t)s * a.nr + r0) * a.nt;
-    for (int i = threadIdx.x; i < ntr; i += blockDim.x)
-        yo[i] = (TD)acc[i];
+    for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
+        if constexpr (FIX)
+            yo[i] = (TD)ldexp((double)(long long)acci[i], -e);
+        else
+            yo[i] = (TD)accd[i];
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+__global__ __launch_bounds__(kSpreadBlock) void spread_kernel(KArgs a, const TD *__restrict__ m,
+                                                              TD *__restrict__ y,
+                                                              const SpreadScale *__restrict__ sc)
+{
+    if constexpr (sizeof(TD) == 4) {
+        const int e = sc->e;
+        if (sc->fixed)
+            spread_body<TD, TT, DYN, NEG, ROT, true, RB>(a, m, y, e);
+        else
+            spread_body<TD, TT, DYN, NEG, ROT, false, RB>(a, m, y, 0);
+    } else {
+        spread_body<TD, TT, DYN, NEG, ROT, false, RB>(a, m, y, 0);
+    }
+}
+
+// Per-launch scale of the fixed-point accumulation: partial max|m| per block
+// (inf for a non-finite value), then one block turns the maximum into e.
+__global__ __launch_bounds__(kBlock) void absmax_partial_kernel(const float *__restrict__ m,
+                                                                int64_t n,
+                                                                float *__restrict__ part)
+{
+    float mx = 0.f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float v = fabsf(m[i]);
+        if (!(v <= 3.402823466e38f))
+            v = INFINITY;  // NaN or inf
+        mx = fmaxf(mx, v);
+    }
+    __shared__ float wmax[kBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        wmax[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = wmax[0];
+        for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
+            b = fmaxf(b, wmax[i]);
+        part[blockIdx.x] = b;
+    }
+}
+
+__global__ __launch_bounds__(kWave) void scale_final_kernel(const float *__restrict__ part, int np,
+                                                            double wamp,
+                                                            SpreadScale *__restrict__ sc)
+{
+    float mx = 0.f;
+    for (int i = threadIdx.x; i < np; i += kWave)
+        mx = fmaxf(mx, part[i]);
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+    if (threadIdx.x == 0) {
+        // a contribution is at most |m| * wamp * 2 (the truncation path of a
+        // negative time weighs a sample by up to 2)
+        const double bound = 2.0 * (double)mx * wamp;
+        if (!(bound <= 1.7976931348623157e308)) {
+            sc->e = 0;
+            sc->fixed = 0;
+        } else if (bound == 0.0) {
+            sc->e = 0;
+            sc->fixed = 1;
+        } else {
+            int ex;
+            frexp(bound, &ex);  // bound < 2^ex
+            sc->e = 30 - ex;
+            sc->fixed = 1;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -328,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(KArgs a, const TD *__res
 #pragma unroll
         for (int j = 0; j < SB; ++j) {
             if (j < nsc) {
-                const size_t so = (size_t)(s0 + j) * a.ntiles + t;
+                const size_t so = (size_t)t * a.ns + s0 + j;
                 os[j] = (TC)ld_off(offs + so * kWave + lane);
                 bs[j] = a.base_s[so];
                 if constexpr (DYN) {
@@ -342,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(KArgs a, const TD *__res
             }
         }
         for (int r = 0; r < a.nr; ++r) {
-            const size_t ro = (size_t)r * a.ntiles + t;
+            const size_t ro = (size_t)t * a.nr + r;
             const TC orr = (TC)ld_off(offr + ro * kWave + lane);
             const int br = a.base_r[ro];
             TD pr = TD(0), qr = TD(0);
@@ -451,9 +589,10 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(KArgs a, const double *_
     }
     double b = floor(0.5 * (mn + mx) + 0.5);
     b = fmin(fmax(b, -536870912.0), 536870912.0);
-    off[(size_t)w * kWave + lane] = to_tab<TT>(q - b);
+    const size_t o = (size_t)t * n + j;  // tile-major output
+    off[o * kWave + lane] = to_tab<TT>(q - b);
     if (lane == 0)
-        base[w] = (int)b;
+        base[o] = (int)b;
 }
 
 template <typename TD>
@@ -467,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(KArgs a, const double *__r
     const int j = (int)(w / a.ntiles);
     const int t = (int)(w % a.ntiles);
     const TilePoint p = tile_point(a, t, lane);
-    out[(size_t)w * kWave + lane] =
+    out[((size_t)t * n + j) * kWave + lane] =
         p.valid ? (TD)tabT[(size_t)j * ni + point_index(a, p)] : TD(0);
 }
 
@@ -593,60 +732,152 @@ KArgs make_args(const kx_plan *p)
     return a;
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int RB>
-int launch_spread_rb(const KArgs &a, const void *m, void *y, hipStream_t st)
+constexpr int kRblocks[] = {16, 8, 4, 2, 1};
+constexpr int kLdsPerCu = 160 * 1024;
+constexpr int kMaxWavesPerCu = 32;
+
+size_t spread_lds_bytes(int rb, int nt) { return (size_t)rb * nt * 8; }
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+int launch_spread_rb(const KArgs &a, const void *m, void *y, const SpreadScale *sc,
+                     hipStream_t st)
 {
-    const size_t lds = (size_t)RB * a.nt * sizeof(AccT);
+    const size_t lds = spread_lds_bytes(RB, a.nt);
     const int nrb = (a.nr + RB - 1) / RB;
     const int nblk = a.ns * nrb;
-    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, RB>), dim3(nblk), dim3(kBlock), lds, st,
-                       a, reinterpret_cast<const TD *>(m), reinterpret_cast<TD *>(y));
+    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, ROT, RB>), dim3(nblk), dim3(kSpreadBlock),
+                       lds, st, a, reinterpret_cast<const TD *>(m), reinterpret_cast<TD *>(y), sc);
     KX_LAUNCHED("spread");
     return KX_OK;
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG>
-int launch_spread_t(const KArgs &a, const void *m, void *y, hipStream_t st)
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+const void *spread_fn(int rb)
 {
-    switch (a.rblock) {
+    switch (rb) {
     case 16:
-        return launch_spread_rb<TD, TT, DYN, NEG, 16>(a, m, y, st);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 16>);
     case 8:
-        return launch_spread_rb<TD, TT, DYN, NEG, 8>(a, m, y, st);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 8>);
     case 4:
-        return launch_spread_rb<TD, TT, DYN, NEG, 4>(a, m, y, st);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 4>);
     case 2:
-        return launch_spread_rb<TD, TT, DYN, NEG, 2>(a, m, y, st);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 2>);
     default:
-        return launch_spread_rb<TD, TT, DYN, NEG, 1>(a, m, y, st);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, false, 1>);
     }
 }
 
-// receivers per workgroup: a power of two in [1, 16] whose traces fit in LDS
-int pick_rblock(int requested, int nr, int per_trace_bytes)
+// Workgroups of one spread kernel resident per CU (registers, LDS, waves);
+// without a device, the LDS and wave limits alone.
+int spread_per_cu(const void *fn, int rb, int nt)
 {
-    int rb = requested > 0 ? requested : 8;
-    rb = std::min(rb, kLdsBudget / per_trace_bytes);
-    int p2 = 1;
-    while (p2 * 2 <= rb && p2 * 2 <= 16)
-        p2 *= 2;
-    while (p2 > 1 && p2 / 2 >= nr)
-        p2 /= 2;  // do not leave a block mostly empty
-    return p2;
+    const size_t lds = spread_lds_bytes(rb, nt);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kSpreadBlock, lds) != hipSuccess ||
+        n <= 0) {
+        (void)hipGetLastError();
+        n = std::min((int)(kLdsPerCu / lds), kMaxWavesPerCu / kSpreadWaves);
+    }
+    return std::max(n, 1);
 }
+
+// Receivers per workgroup.  A requested value is rounded down to a supported
+// one whose traces fit in LDS.  Automatic: the block that keeps the most
+// traces resident per CU, weighted by how far the resident waves (up to 24)
+// can hide the latency of the table loads; ties go to the larger block,
+// which re-reads the image and the source table less often.
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+int pick_rblock(int requested, int nr, int nt, int max_rb)
+{
+    auto fits = [&](int rb) {
+        return rb <= max_rb && spread_lds_bytes(rb, nt) <= (size_t)kLdsBudget;
+    };
+    if (requested > 0) {
+        for (int rb : kRblocks)
+            if (rb <= requested && fits(rb))
+                return rb;
+        return 1;
+    }
+    int best = 1;
+    double best_score = -1.0;
+    for (int rb : kRblocks) {
+        if (!fits(rb) || (rb > 1 && rb / 2 >= nr))
+            continue;
+        const int per_cu = spread_per_cu(spread_fn<TD, TT, DYN, NEG, ROT>(rb), rb, nt);
+        const double waves = std::min(24.0, (double)per_cu * kSpreadWaves);
+        const double score = waves / 24.0 * per_cu * std::min(rb, nr);
+        if (score > best_score * 1.0001) {
+            best = rb;
+            best_score = score;
+        }
+    }
+    return best;
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+int launch_spread_o(KArgs a, int requested, int max_rb, const void *m, void *y,
+                    const SpreadScale *sc, hipStream_t st)
+{
+    a.rblock = pick_rblock<TD, TT, DYN, NEG, ROT>(requested, a.nr, a.nt, max_rb);
+    switch (a.rblock) {
+    case 16:
+        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 16>(a, m, y, sc, st);
+    case 8:
+        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 8>(a, m, y, sc, st);
+    case 4:
+        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 4>(a, m, y, sc, st);
+    case 2:
+        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 2>(a, m, y, sc, st);
+    default:
+        return launch_spread_rb<TD, TT, DYN, NEG, false, 1>(a, m, y, sc, st);
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG>
+int launch_spread_t(const KArgs &a, int order, int requested, int max_rb, const void *m, void *y,
+                    const SpreadScale *sc, hipStream_t st)
+{
+    return order == KX_ORDER_ROTATED
+               ? launch_spread_o<TD, TT, DYN, NEG, true>(a, requested, max_rb, m, y, sc, st)
+               : launch_spread_o<TD, TT, DYN, NEG, false>(a, requested, max_rb, m, y, sc, st);
+}
+
 
 template <typename TD, typename TT>
 int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream_t st)
 {
-    const int per_trace = p->nt * (int)sizeof(AccT);
-    if (per_trace > kLdsBudget)
+    if (spread_lds_bytes(1, p->nt) > (size_t)kLdsBudget)
         return kx_fail(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
-    a.rblock = pick_rblock(p->rblock, p->nr, per_trace);
+    if (sizeof(TD) == 4 && !p->aux)
+        return kx_fail(KX_EINVAL, "spread: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
+    // registers per lane grow with the block: keep two workgroups per CU
+    const int max_rb = sizeof(TD) == 8 ? 4 : (p->dynamic ? 8 : 16);
+    SpreadScale *sc = reinterpret_cast<SpreadScale *>(p->aux);
+    if constexpr (sizeof(TD) == 4) {
+        // fixed-point scale of this launch from max|m|
+        const int64_t ni = (int64_t)p->nx * p->ny * p->nz;
+        float *part = reinterpret_cast<float *>(p->aux) + 16;
+        const int nb = (int)std::min<int64_t>(kScaleBlocks, (ni + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(absmax_partial_kernel, dim3(nb), dim3(kBlock), 0, st,
+                           reinterpret_cast<const float *>(m), ni, part);
+        KX_LAUNCHED("spread scale");
+        hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp, sc);
+        KX_LAUNCHED("spread scale");
+    }
+    // automatic lane order: rotated when a step loads one receiver table,
+    // natural when it loads three (dynamic weights), where the scattered
+    // loads cost more than the collisions they avoid
+    int order = p->sorder;
+    if (order == KX_ORDER_AUTO)
+        order = p->dynamic ? KX_ORDER_NATURAL : KX_ORDER_ROTATED;
     if (p->dynamic)
-        return p->neg_trav ? launch_spread_t<TD, TT, true, true>(a, m, y, st)
-                           : launch_spread_t<TD, TT, true, false>(a, m, y, st);
-    return p->neg_trav ? launch_spread_t<TD, TT, false, true>(a, m, y, st)
-                       : launch_spread_t<TD, TT, false, false>(a, m, y, st);
+        return p->neg_trav
+                   ? launch_spread_t<TD, TT, true, true>(a, order, p->rblock, max_rb, m, y, sc, st)
+                   : launch_spread_t<TD, TT, true, false>(a, order, p->rblock, max_rb, m, y, sc, st);
+    return p->neg_trav
+               ? launch_spread_t<TD, TT, false, true>(a, order, p->rblock, max_rb, m, y, sc, st)
+               : launch_spread_t<TD, TT, false, false>(a, order, p->rblock, max_rb, m, y, sc, st);
 }
 
 template <typename TD, typename TT, bool DYN, bool NEG>

@@ -201,8 +201,8 @@ def encode(plan, tab, scale, fmt):
     tabT = tab.t().contiguous()
     tdt = {_lib.KX_F32: torch.float32, _lib.KX_F16: torch.float16,
            _lib.KX_F64: torch.float64}[fmt]
-    off = torch.empty((n, plan.ntiles, 64), dtype=tdt, device=tab.device)
-    base = torch.empty((n, plan.ntiles), dtype=torch.int32, device=tab.device)
+    off = torch.empty((plan.ntiles, n, 64), dtype=tdt, device=tab.device)
+    base = torch.empty((plan.ntiles, n), dtype=torch.int32, device=tab.device)
     rc = _lib.load().kx_encode_table(ctypes.byref(plan), _lib.ptr(tabT), n, float(scale), fmt,
                                      _lib.ptr(off), _lib.ptr(base), _lib.stream_ptr(tab.device))
     _lib.check(rc, "kx_encode_table")
@@ -210,12 +210,24 @@ def encode(plan, tab, scale, fmt):
 
 
 def tile(plan, tab, fmt):
-    """float64 (ni, n) -> [n][ntiles][64] of dtype fmt, zero padded."""
+    """float64 (ni, n) -> [ntiles][n][64] of dtype fmt, zero padded."""
     n = tab.shape[1]
     tabT = tab.t().contiguous()
     tdt = torch.float64 if fmt == _lib.KX_F64 else torch.float32
-    out = torch.empty((n, plan.ntiles, 64), dtype=tdt, device=tab.device)
+    out = torch.empty((plan.ntiles, n, 64), dtype=tdt, device=tab.device)
     rc = _lib.load().kx_tile_table(ctypes.byref(plan), _lib.ptr(tabT), n, fmt, _lib.ptr(out),
                                    _lib.stream_ptr(tab.device))
     _lib.check(rc, "kx_tile_table")
     return out
+
+
+def weight_bound(wp, wq):
+    """max over the table of |(wp, wq)|, in float64 (device tensors of one
+    side's weight tables in the tiled layout)."""
+    wp, wq = wp.reshape(-1), wq.reshape(-1)
+    best = torch.zeros((), dtype=torch.float64, device=wp.device)
+    step = 1 << 26
+    for i in range(0, wp.numel(), step):
+        a, b = wp[i:i + step].double(), wq[i:i + step].double()
+        best = torch.maximum(best, torch.sqrt(a * a + b * b).max())
+    return float(best)

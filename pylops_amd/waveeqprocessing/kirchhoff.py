@@ -332,8 +332,15 @@ class Kirchhoff(LinearOperator):
             keep["wq_s"] = T.tile(plan, qs, wfmt)
             keep["wp_r"] = T.tile(plan, pr, wfmt)
             keep["wq_r"] = T.tile(plan, qr, wfmt)
+            # |ps pr + qs qr| <= |(ps, qs)| |(pr, qr)|: bounds every weight
+            plan.wamp = T.weight_bound(keep["wp_s"], keep["wq_s"]) * \
+                T.weight_bound(keep["wp_r"], keep["wq_r"])
             izmin = T.aperture_izmin(self.six, self.rix, self.nz, float(self.aperture[1]))
             keep["izmin"] = torch.as_tensor(izmin, device=dev)
+        else:
+            plan.wamp = 1.0
+        # forward scratch (accumulation scale), written by every forward
+        keep["aux"] = torch.zeros(_lib.KX_AUX_BYTES // 4, dtype=torch.int32, device=dev)
         tdt = torch.float64 if self.dtype == np.float64 else torch.float32
         keep["h"] = torch.as_tensor(np.asarray(self.cop.h, dtype=np.float64), dtype=tdt,
                                     device=dev)

@@ -57,6 +57,33 @@ def test_plan_struct_layout_matches_header(tmp_path):
         assert getattr(_lib.KxPlan, f).offset == off, f
 
 
+def test_header_constants_match_binding():
+    from pylops_amd import _lib
+    text = open(HEADER).read()
+    for name in ("KX_ABI_VERSION", "KX_AUX_BYTES", "KX_F32", "KX_F64", "KX_F16", "KX_ORDER_AUTO",
+                 "KX_ORDER_NATURAL", "KX_ORDER_ROTATED",
+                 "KX_EINVAL", "KX_ESHAPE", "KX_ETYPE", "KX_EABI", "KX_ELDS"):
+        m = re.search(rf"#define {name} (-?\d+)", text)
+        assert m is not None, name
+        assert int(m.group(1)) == getattr(_lib, name), name
+
+
+def test_spread_requires_aux():
+    """float data needs the caller's aux scratch; the check precedes any
+    device call."""
+    import ctypes
+    from pylops_amd import _lib
+    from pylops_amd.waveeqprocessing import _tables as T
+    L = _lib.load()
+    p = T.make_geom_plan(8, 1, 16, (16, 4, 1))
+    p.dtype = p.tab_fmt = _lib.KX_F32
+    p.ns, p.nr, p.nt = 1, 1, 16
+    p.off_s = p.base_s = p.off_r = p.base_r = 16  # never dereferenced
+    rc = L.kx_spread(ctypes.byref(p), ctypes.c_void_p(16), ctypes.c_void_p(16), None)
+    assert rc == _lib.KX_EINVAL
+    assert "aux" in L.kx_last_error().decode()
+
+
 def test_error_mapping():
     from pylops_amd import _lib
     import ctypes

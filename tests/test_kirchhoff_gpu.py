@@ -202,3 +202,54 @@ def test_dot_fp64_accumulation():
         _lib.check(rc)
         ref = np.dot(a.astype(np.float64), b.astype(np.float64))
         assert abs(float(out) - ref) <= 1e-12 * max(1.0, np.abs(a).dot(np.abs(b)))
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_spread_lane_orders_and_receiver_blocks(dynamic):
+    """Every receiver block size and lane order of the forward, with a last
+    block that is only partly filled (11 receivers), matches the oracle; the
+    fixed-point sums do not depend on the order of the adds, so all
+    variants agree bit for bit."""
+    from pylops_amd import _lib
+    recs = np.vstack((np.linspace(G.x.min(), G.x.max(), 11), 2 * np.ones(11)))
+    srcs = np.vstack((np.linspace(G.x.min(), G.x.max(), 3), 2 * np.ones(3)))
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=srcs, recs=recs, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dynamic=dynamic, dtype="float32",
+              angleaperture=60)
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    m = np.random.default_rng(9).standard_normal(Op.shape[1]).astype(np.float32)
+    want = orc.spread(m)
+    outs = []
+    for order in (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED):
+        for rb in (1, 2, 4, 8, 16):
+            Op.plan.sorder = order
+            Op.plan.rblock = rb
+            got = np.asarray(Op.spread(m))
+            assert G.relerr(got, want) < 1e-5, (order, rb)
+            outs.append(got)
+    for got in outs[1:]:
+        np.testing.assert_array_equal(got, outs[0])
+
+
+def test_spread_nonfinite_model_propagates_like_reference():
+    """inf / NaN in the model switch the launch to float64 accumulation:
+    the samples they reach become non-finite exactly where the oracle's
+    do, the others keep oracle parity."""
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dynamic=False, dtype="float32")
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    m = np.random.default_rng(3).standard_normal(Op.shape[1]).astype(np.float32)
+    m[17] = np.nan
+    m[Op.shape[1] // 2] = np.inf
+    got = np.asarray(Op.spread(m))
+    want = orc.spread(m)
+    bad = ~np.isfinite(want)
+    assert bad.any()
+    np.testing.assert_array_equal(~np.isfinite(got), bad)
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    assert G.relerr(got[~bad], want[~bad]) < 1e-5
+    # a finite model right after still takes the fixed-point path
+    m2 = np.nan_to_num(m, nan=0.0, posinf=0.0)
+    assert G.relerr(np.asarray(Op.spread(m2)), orc.spread(m2)) < 1e-5
