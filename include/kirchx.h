@@ -100,6 +100,9 @@ typedef struct kx_plan {
     void *aux;    /* KX_AUX_BYTES of device memory owned by the caller; the
                      forward writes its accumulation scale there, so calls that
                      share one aux must be ordered on one stream */
+    int32_t gsplit; /* waves per tile in the adjoint, 1, 2 or 4 (each sums a
+                       share of the sources), 0 = automatic */
+    int32_t pad1;
 } kx_plan;
 
 int kx_version(void);

@@ -120,6 +120,8 @@ class KxPlan(ctypes.Structure):
         ("hstar", ctypes.c_void_p),
         ("wamp", ctypes.c_double),
         ("aux", ctypes.c_void_p),
+        ("gsplit", ctypes.c_int32),
+        ("pad1", ctypes.c_int32),
     ]
 
 

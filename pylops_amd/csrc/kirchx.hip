@@ -82,6 +82,10 @@ constexpr int kReduceBlocks = 1024;
 template <typename TD> struct TimeType { using type = float; };
 template <> struct TimeType<double> { using type = double; };
 
+template <typename T, int N> struct alignas(sizeof(T) * N) VecN {
+    T v[N];
+};
+
 __device__ __forceinline__ float ld_off(const float *p) { return *p; }
 __device__ __forceinline__ float ld_off(const __half *p) { return __half2float(*p); }
 __device__ __forceinline__ double ld_off(const double *p) { return *p; }
@@ -435,19 +439,38 @@ __global__ __launch_bounds__(kWave) void scale_final_kernel(const float *__restr
 }
 
 // ---------------------------------------------------------------------------
-// adjoint: gather per image point
+// adjoint: gather per image point from wave-staged trace windows
 // ---------------------------------------------------------------------------
-template <typename TD, typename TT, bool DYN, bool NEG, int SB>
-__global__ __launch_bounds__(kBlock) void gather_kernel(KArgs a, const TD *__restrict__ d,
-                                                        TD *__restrict__ m)
+// A wave owns one tile and a share of the sources (P waves of a workgroup
+// split a tile's sources; their partial sums are added in a fixed order at
+// the end).  It sweeps the receivers r; for each of its JW sources the trace
+// window of kWin samples centred on the tile's base sum (source plus
+// receiver base: the tile's own offsets span a few tens of samples) is read
+// with one coalesced load -- issued a round ahead -- into the wave's own LDS
+// region, and each lane reads its two samples there.  A 64-lane gather from
+// global memory costs the texture path ~10x the coalesced load and LDS reads
+// that replace it.  Samples outside the window are read from global memory,
+// so the window size affects speed only.  No barrier until the final sum.
+constexpr int kGatherWaves = 4;
+constexpr int kGatherBlock = kGatherWaves * kWave;
+constexpr int kWin = kWave;  // samples per staged window, one per lane
+
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+__global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD *__restrict__ d,
+                                                              TD *__restrict__ m)
 {
     using TC = typename TimeType<TD>::type;
+    constexpr int NT = kGatherWaves / P;  // tiles per workgroup
+    constexpr int SB = JW * P;            // sources per block (all parts)
+    __shared__ TD win[kGatherWaves][JW][kWin];
+
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int part = wave / NT;  // this wave sums sources s0 + part + P i
     const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int t = L * (kBlock / kWave) + wave;
-    if (t >= a.ntiles)
-        return;  // no barrier in this kernel
+    const int tt = L * NT + wave % NT;
+    const bool active = tt < a.ntiles;
+    const int t = active ? tt : a.ntiles - 1;
     const TilePoint p = tile_point(a, t, lane);
     const TT *offs = reinterpret_cast<const TT *>(a.off_s);
     const TT *offr = reinterpret_cast<const TT *>(a.off_r);
@@ -456,68 +479,133 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(KArgs a, const TD *__res
     const TD *wpr = reinterpret_cast<const TD *>(a.wp_r);
     const TD *wqr = reinterpret_cast<const TD *>(a.wq_r);
     const unsigned ntm1 = (unsigned)(a.nt - 1);
+    const int vz = vgpr_zero();
+    TD(&w)[JW][kWin] = win[wave];
 
     TD acc = TD(0);
-    for (int s0 = 0; s0 < a.ns; s0 += SB) {
-        const int nsc = min(SB, a.ns - s0);
-        TC os[SB];
-        int bs[SB];
-        TD ps[SB], qs[SB];
+    if (active) {
+        for (int s0 = 0; s0 < a.ns; s0 += SB) {
+            // this wave's sources j = s0 + part + P i
+            TC os[JW];
+            int bs[JW];
+            TD ps[JW], qs[JW];
+            // trace (j, receiver 0) as an element offset from the block's
+            // first trace (fits 31 bits: checked at launch), held in vector
+            // registers: JW 64-bit pointers would exhaust the scalar ones
+            const TD *dblk = d + (size_t)s0 * a.nr * a.nt;
+            int soff[JW], izoff[JW];
 #pragma unroll
-        for (int j = 0; j < SB; ++j) {
-            if (j < nsc) {
-                const size_t so = (size_t)t * a.ns + s0 + j;
-                os[j] = (TC)ld_off(offs + so * kWave + lane);
-                bs[j] = a.base_s[so];
-                if constexpr (DYN) {
-                    ps[j] = wps[so * kWave + lane];
-                    qs[j] = wqs[so * kWave + lane];
+            for (int i = 0; i < JW; ++i) {
+                const int j = s0 + part + P * i;
+                if (j < a.ns) {
+                    const size_t so = (size_t)t * a.ns + j;
+                    os[i] = (TC)ld_off(offs + so * kWave + lane);
+                    bs[i] = a.base_s[so + vz];
+                    if constexpr (DYN) {
+                        ps[i] = wps[so * kWave + lane];
+                        qs[i] = wqs[so * kWave + lane];
+                    }
+                } else {  // past the last source: every sample before the trace
+                    os[i] = TC(0);
+                    bs[i] = -(1 << 30);
+                    ps[i] = qs[i] = TD(0);
                 }
-            } else {
-                os[j] = TC(0);
-                bs[j] = 0;
-                ps[j] = qs[j] = TD(0);
+                soff[i] = (min(j, a.ns - 1) - s0) * a.nr * a.nt + vz;
+                izoff[i] = min(j, a.ns - 1) * a.nr + vz;
             }
-        }
-        for (int r = 0; r < a.nr; ++r) {
-            const size_t ro = (size_t)t * a.nr + r;
-            const TC orr = (TC)ld_off(offr + ro * kWave + lane);
-            const int br = a.base_r[ro];
-            TD pr = TD(0), qr = TD(0);
+
+            // the window of source i in round r starts at bs + br(r) - kWin / 2;
+            // lane l stages its sample l
+            TD sv[JW];
+#define KX_STAGE(R, BR)                                                                        \
+    _Pragma("unroll") for (int i = 0; i < JW; ++i)                                              \
+    {                                                                                          \
+        const int e = bs[i] + (BR) - kWin / 2 + lane;                                          \
+        sv[i] = (unsigned)e < (unsigned)a.nt ? dblk[soff[i] + (R) * a.nt + e] : TD(0);          \
+    }
+            // receiver side of round r, and receiver base of round r + 1
+            TT orr_c = offr[((size_t)t * a.nr) * kWave + lane];
+            int br_c = a.base_r[(size_t)t * a.nr + vz];
+            TD pr_c = TD(0), qr_c = TD(0);
             if constexpr (DYN) {
-                pr = wpr[ro * kWave + lane];
-                qr = wqr[ro * kWave + lane];
+                pr_c = wpr[((size_t)t * a.nr) * kWave + lane];
+                qr_c = wqr[((size_t)t * a.nr) * kWave + lane];
             }
+            KX_STAGE(0, br_c)
+            for (int r = 0; r < a.nr; ++r) {
+                // this round's samples into the wave's LDS region
 #pragma unroll
-            for (int j = 0; j < SB; ++j) {
-                if (j < nsc) {
-                    const TC T = os[j] + orr;
+                for (int i = 0; i < JW; ++i)
+                    w[i][lane] = sv[i];
+                const TC orr = (TC)ld_off(&orr_c);
+                const int br = br_c;
+                const TD pr = pr_c, qr = qr_c;
+                // next round's receiver side and samples, in flight meanwhile
+                if (r + 1 < a.nr) {
+                    const size_t ro = (size_t)t * a.nr + r + 1;
+                    orr_c = offr[ro * kWave + lane];
+                    br_c = a.base_r[ro + vz];
+                    if constexpr (DYN) {
+                        pr_c = wpr[ro * kWave + lane];
+                        qr_c = wqr[ro * kWave + lane];
+                    }
+                    KX_STAGE(r + 1, br_c)
+                }
+#pragma unroll
+                for (int i = 0; i < JW; ++i) {
+                    const TC T = os[i] + orr;
                     const TC fl = floor(T);
                     TC f = T - fl;
-                    int it = (int)fl + bs[j] + br;
+                    int it = (int)fl + bs[i] + br;
                     if constexpr (NEG) {
+                        // the reference truncates toward zero: a time in
+                        // (-1, 0) reads sample 0 with a negative weight
                         if (it == -1 && f > TC(0)) {
                             it = 0;
                             f -= TC(1);
                         }
                     }
-                    const int trace = (s0 + j) * a.nr + r;
-                    if ((unsigned)it < ntm1) {
-                        const TD *q = d + (size_t)trace * a.nt + it;
-                        TD v = q[0] * (TD)(TC(1) - f) + q[1] * (TD)f;
-                        if constexpr (DYN) {
-                            TD w = ps[j] * pr + qs[j] * qr;
-                            if (p.iz < a.izmin[trace])
-                                w = TD(0);
-                            v *= w;
-                        }
-                        acc += v;
+                    // The LDS read is unconditional (clamped index); a sample
+                    // outside the window overwrites it from global memory (a
+                    // branch between the two loads would merge them into one
+                    // generic-address load).
+                    const int kk = it - (bs[i] + br - kWin / 2);
+                    const int kc = min(max(kk, 0), kWin - 2);
+                    TD q0 = w[i][kc];
+                    TD q1 = w[i][kc + 1];
+                    const bool in_trace = (unsigned)it < ntm1;
+                    if (in_trace && kk != kc) {
+                        const TD *q = dblk + (soff[i] + r * a.nt + it);
+                        q0 = q[0];
+                        q1 = q[1];
                     }
+                    TD v = q0 * (TD)(TC(1) - f) + q1 * (TD)f;
+                    if constexpr (DYN) {
+                        TD wgt = ps[i] * pr + qs[i] * qr;
+                        if (p.iz < a.izmin[izoff[i] + r])
+                            wgt = TD(0);
+                        v *= wgt;
+                    }
+                    acc += in_trace ? v : TD(0);
                 }
             }
+#undef KX_STAGE
         }
     }
-    if (p.valid)
+    if constexpr (P > 1) {
+        // the partial sums of a tile's points, added in part order
+        __syncthreads();
+        TD *red = &win[0][0][0];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (part == 0) {
+            acc = TD(0);
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+                acc += red[(q * NT + wave) * kWave + lane];
+        }
+    }
+    if (part == 0 && active && p.valid)
         m[point_index(a, p)] = acc;
 }
 
@@ -880,25 +968,59 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
                : launch_spread_t<TD, TT, false, false>(a, order, p->rblock, max_rb, m, y, sc, st);
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG>
-int launch_gather_t(const KArgs &a, const void *d, void *m, hipStream_t st)
+template <typename TD, typename TT, bool DYN, bool NEG, int P>
+int launch_gather_p(const KArgs &a, const void *d, void *m, hipStream_t st)
 {
-    constexpr int SB = sizeof(TD) == 8 ? 8 : 16;
-    const int nblk = (a.ntiles + (kBlock / kWave) - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, SB>), dim3(nblk), dim3(kBlock), 0, st, a,
-                       reinterpret_cast<const TD *>(d), reinterpret_cast<TD *>(m));
+    // sources per wave and pass: every pass re-reads the receiver table, but
+    // more sources hold more registers (the dynamic weights two per source)
+    constexpr int JW = DYN ? 4 : 8;
+    constexpr int NT = kGatherWaves / P;
+    if ((int64_t)JW * P * a.nr * a.nt >= ((int64_t)1 << 31))
+        return kx_fail(KX_ESHAPE, "gather: %d sources x nr x nt exceeds 2^31 samples", JW * P);
+    const int nblk = (a.ntiles + NT - 1) / NT;
+    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P>), dim3(nblk), dim3(kGatherBlock),
+                       0, st, a, reinterpret_cast<const TD *>(d), reinterpret_cast<TD *>(m));
     KX_LAUNCHED("gather");
     return KX_OK;
+}
+
+// Waves per tile: the fewest that give every CU kGatherFill waves.
+constexpr int kGatherFill = 24;
+
+template <typename TD, typename TT, bool DYN, bool NEG>
+int launch_gather_t(const KArgs &a, int gsplit, const void *d, void *m, hipStream_t st)
+{
+    int P = 1;
+    if (gsplit > 0) {
+        while (P * 2 <= std::min(gsplit, kGatherWaves))
+            P *= 2;
+    } else {
+        int cus = 256, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        while (P < kGatherWaves && (int64_t)a.ntiles * P < (int64_t)kGatherFill * cus)
+            P *= 2;
+    }
+    switch (P) {
+    case 1:
+        return launch_gather_p<TD, TT, DYN, NEG, 1>(a, d, m, st);
+    case 2:
+        return launch_gather_p<TD, TT, DYN, NEG, 2>(a, d, m, st);
+    default:
+        return launch_gather_p<TD, TT, DYN, NEG, 4>(a, d, m, st);
+    }
 }
 
 template <typename TD, typename TT>
 int launch_gather_d(const kx_plan *p, const KArgs &a, const void *d, void *m, hipStream_t st)
 {
+    const int g = p->gsplit;
     if (p->dynamic)
-        return p->neg_trav ? launch_gather_t<TD, TT, true, true>(a, d, m, st)
-                           : launch_gather_t<TD, TT, true, false>(a, d, m, st);
-    return p->neg_trav ? launch_gather_t<TD, TT, false, true>(a, d, m, st)
-                       : launch_gather_t<TD, TT, false, false>(a, d, m, st);
+        return p->neg_trav ? launch_gather_t<TD, TT, true, true>(a, g, d, m, st)
+                           : launch_gather_t<TD, TT, true, false>(a, g, d, m, st);
+    return p->neg_trav ? launch_gather_t<TD, TT, false, true>(a, g, d, m, st)
+                       : launch_gather_t<TD, TT, false, false>(a, g, d, m, st);
 }
 
 }  // namespace

@@ -231,3 +231,4 @@ def weight_bound(wp, wq):
         a, b = wp[i:i + step].double(), wq[i:i + step].double()
         best = torch.maximum(best, torch.sqrt(a * a + b * b).max())
     return float(best)
+

@@ -253,3 +253,43 @@ def test_spread_nonfinite_model_propagates_like_reference():
     # a finite model right after still takes the fixed-point path
     m2 = np.nan_to_num(m, nan=0.0, posinf=0.0)
     assert G.relerr(np.asarray(Op.spread(m2)), orc.spread(m2)) < 1e-5
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_gather_source_splits(dynamic, dtype):
+    """Every split of a tile's sources over waves (kx_plan.gsplit) matches
+    the oracle; 19 sources leave a partly filled source block."""
+    srcs = np.vstack((np.linspace(G.x.min(), G.x.max(), 19), 2 * np.ones(19)))
+    recs = np.vstack((np.linspace(G.x.min(), G.x.max(), 13), 2 * np.ones(13)))
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=srcs, recs=recs, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dynamic=dynamic, dtype=dtype,
+              angleaperture=60)
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    d = np.random.default_rng(4).standard_normal(Op.shape[0]).astype(dtype)
+    want = orc.gather(d)
+    tol = 1e-5 if dtype == "float32" else 1e-10
+    for split in (1, 2, 4):
+        Op.plan.gsplit = split
+        assert G.relerr(np.asarray(Op.gather(d)), want) < tol, split
+    Op.plan.gsplit = 0
+
+
+@pytest.mark.parametrize("split", [1, 4])
+def test_gather_samples_outside_the_staged_window(split):
+    """A workgroup's tiles whose traveltimes span far more than the staged
+    window (steep, finely sampled times over a deep image) read the samples
+    outside it from global memory."""
+    z = np.arange(256) * 4.0
+    x = np.arange(6) * 4.0
+    t = np.arange(2400) * 0.0005
+    srcs = np.vstack((np.array([0.0, 20.0]), np.zeros(2)))
+    recs = np.vstack((np.array([4.0, 12.0, 20.0]), np.zeros(3)))
+    kw = dict(z=z, x=x, t=t, srcs=srcs, recs=recs, vel=2000.0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dynamic=False, dtype="float32")
+    Op = _K(**kw, tile=(16, 4, 1))
+    orc = O.KirchhoffOracle(**kw)
+    Op.plan.gsplit = split
+    d = np.random.default_rng(6).standard_normal(Op.shape[0]).astype(np.float32)
+    assert G.relerr(np.asarray(Op.gather(d)), orc.gather(d)) < 1e-5
