@@ -971,9 +971,10 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
 template <typename TD, typename TT, bool DYN, bool NEG, int P>
 int launch_gather_p(const KArgs &a, const void *d, void *m, hipStream_t st)
 {
-    // sources per wave and pass: every pass re-reads the receiver table, but
-    // more sources hold more registers (the dynamic weights two per source)
-    constexpr int JW = DYN ? 4 : 8;
+    // sources per wave and pass (measured): every pass re-reads the receiver
+    // table -- three tables when dynamic -- and more sources hold more
+    // registers, lowering occupancy
+    constexpr int JW = DYN ? 8 : 4;
     constexpr int NT = kGatherWaves / P;
     if ((int64_t)JW * P * a.nr * a.nt >= ((int64_t)1 << 31))
         return kx_fail(KX_ESHAPE, "gather: %d sources x nr x nt exceeds 2^31 samples", JW * P);
