@@ -150,6 +150,22 @@ def workload_2dsmall(rng):
     return kw, refl, desc
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` (float32 tables) from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries in profiles/
+    (a 'TRAFFIC <workload> <kernel> <bytes>' line), or None."""
+    import glob
+    import re
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "*.txt"))):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"TRAFFIC (\S+) (\S+) (\d+)", line)
+                if m and m.group(1) == workload and m.group(2) == kernel:
+                    return int(m.group(3))
+    return None
+
+
 def algorithmic_bytes(desc, ni, table_bytes, dtype_bytes=4):
     """Separable-stream model: the receiver table streamed once per source,
     source tables once, the image once and the data once; dynamic tables
@@ -318,7 +334,9 @@ def main():
                                  "gather": round(t_gather * 1e3, 4)}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (measured_traffic(args.workload, dom)
+                                 if args.tables == "f32" else None),
                      "model_bytes": int(bytes_alg)},
     }
     if not args.no_solve:
