@@ -1,7 +1,7 @@
 """Kirchhoff forward + adjoint throughput on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2d|3d|2dsmall]
-                    [--tables f32|f16] [--no-cpu] [--extra]
+                    [--tables f32|f16] [--no-cpu] [--extra] [--no-solve] [--no-batch]
 
 A step is one forward (spread + wavelet convolution) of a dense image and
 one adjoint (wavelet correlation + gather) of the observed data, i.e. the
@@ -232,6 +232,8 @@ def main():
                     help="also time the 3D workload with float32 and float16 tables")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip timing of the 30-iteration CGLS / LSQR solves")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batch of 64 uneven 2D problems (batched CGLS)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 4 GiB device copy first (reference kernel for "
                          "FETCH_SIZE / WRITE_SIZE counter scaling)")
@@ -343,6 +345,8 @@ def main():
         out["solve"] = time_solvers(Op, dobs)
     if args.extra:
         out["extra"] = extra_3d(dev)
+    if not args.no_batch:
+        out["batch"] = time_batch(dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(kw, desc, refl, dobs.cpu().numpy())
     print(json.dumps(out), flush=True)
@@ -366,6 +370,98 @@ def time_solvers(Op, dobs, niter=30):
         res[f"{name}_s_per_solve"] = round(dt, 4)
         res[f"{name}_iters"] = int(r[2])
     return res
+
+
+def workload_batch(rng, nprob=64):
+    """64 independent 2D target-oriented LSM problems of uneven sizes: nz
+    U[64,256], nx U[64,512] at 10 m, nsrc U[4,32], nrec U[32,256] on the
+    surface, nt U[256,1024] at 4 ms, constant velocity U[1500,3500] m/s,
+    15-30 Hz Ricker; reflectivity = random layers plus 1% sparse spikes."""
+    probs = []
+    for _ in range(nprob):
+        nz = int(rng.integers(64, 257))
+        nx = int(rng.integers(64, 513))
+        ns = int(rng.integers(4, 33))
+        nr = int(rng.integers(32, 257))
+        nt = int(rng.integers(256, 1025))
+        x = np.arange(nx) * 10.0
+        z = np.arange(nz) * 10.0
+        t = np.arange(nt) * 0.004
+        srcs = np.vstack((np.linspace(x[0], x[-1], ns), np.zeros(ns)))
+        recs = np.vstack((np.linspace(x[0], x[-1], nr), np.zeros(nr)))
+        wav, wc = ricker(t[:41], float(rng.uniform(15.0, 30.0)))
+        kw = dict(z=z, x=x, t=t, srcs=srcs, recs=recs, vel=float(rng.uniform(1500.0, 3500.0)),
+                  wav=wav.astype(np.float32), wavcenter=wc, mode="analytic", dynamic=False)
+        refl = np.zeros((nx, nz), dtype=np.float32)
+        nl = int(rng.integers(3, 9))
+        for iz, a in zip(np.sort(rng.choice(np.arange(8, nz - 4), nl, replace=False)),
+                         rng.uniform(-1, 1, nl)):
+            refl[:, iz] = a
+        spikes = rng.random((nx, nz)) < 0.01
+        refl[spikes] += rng.uniform(-1, 1, int(spikes.sum())).astype(np.float32)
+        probs.append((kw, refl))
+    return probs
+
+
+def time_batch(dev, niter=20):
+    """The batch of 64 uneven problems with fp16 tables: one batched
+    forward + adjoint (persistent launches), and 20 CGLS iterations per
+    problem, all problems at once (graph-replayed iterations) and, for
+    comparison, one problem after the other with the single-problem
+    solver."""
+    import torch
+    import pylops_amd
+    from pylops_amd.optimization.basic import cgls, cgls_batch
+    from pylops_amd.waveeqprocessing import KirchhoffBatch
+    rng = np.random.default_rng(1004)
+    probs = workload_batch(rng)
+    t0 = time.perf_counter()
+    ops = [pylops_amd.Kirchhoff(dtype="float32", tabledtype="float16", **kw) for kw, _ in probs]
+    B = KirchhoffBatch(ops)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    m = torch.as_tensor(np.concatenate([r.ravel() for _, r in probs]), device=dev)
+    y = torch.empty(B.shape[0], dtype=torch.float32, device=dev)
+    a = torch.empty(B.shape[1], dtype=torch.float32, device=dev)
+    B._into(m, y, False)
+    B._into(y, a, True)
+    torch.cuda.synchronize()
+    n = 5
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(n):
+        B._into(a, y, False)
+        B._into(y, a, True)
+    e1.record()
+    torch.cuda.synchronize()
+    t_pair = e0.elapsed_time(e1) / n * 1e-3
+    B._into(m, y, False)
+    trip = float(sum(op.ni * op.ns * op.nr for op in ops))
+    cgls_batch(B, y, niter=2, tol=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = cgls_batch(B, y, niter=niter, tol=0.0)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    ys = B.split_data(y)
+    t0 = time.perf_counter()
+    seq = [cgls(op, yp, niter=niter, tol=0.0, graph=True) for op, yp in zip(ops, ys)]
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    diff = max(float((rb[0].reshape(-1) - rs[0].reshape(-1)).norm() / rs[0].norm())
+               for rb, rs in zip(res, seq))
+    return {"workload": "lsm_batch_64_2d_uneven", "nprob": len(ops), "tables": "f16",
+            "niter": niter, "rblock": B.rblock, "gsplit": B.gsplit,
+            "spread_items": int(B._batch.n_spread), "gather_items": int(B._batch.n_gather),
+            "model_points": int(B.shape[1]), "data_samples": int(B.shape[0]),
+            "setup_s": round(t_setup, 3),
+            "fwd_adj_ms": round(t_pair * 1e3, 3),
+            "fwd_adj_gsamples_s": round(2 * trip / t_pair / 1e9, 2),
+            "cgls_s_per_batch_solve": round(t_batch, 4),
+            "cgls_s_sequential_solves": round(t_seq, 4),
+            "speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "max_rel_diff_batch_vs_sequential_x": diff,
+            "iters": sorted(set(int(r[2]) for r in res))}
 
 
 def extra_3d(dev):

@@ -243,6 +243,73 @@ int kx_lsqr_xw(int64_t n, int32_t dtype, void *x, void *w, const void *v, double
 int kx_cgls_scalar(double *sc, double *cost, double *cost1, int32_t phase, void *stream);
 int kx_lsqr_scalar(double *sc, double *cost, int32_t phase, void *stream);
 
+/* ---- batches of independent problems ----
+ * Replace the reference's one-problem-at-a-time use of Kirchhoff._matvec /
+ * _rmatvec (K:1048-1143) and CGLS.step (CB:510-564) for many problems: every
+ * operator application of a batch is one persistent launch over all
+ * problems' work, and the CGLS vector algebra runs segment-wise, one segment
+ * per problem, with per-problem float64 scalars.  Every problem of a batch
+ * shares dtype, tab_fmt, dynamic and neg_trav; models and data are
+ * concatenated in problem order. */
+typedef struct kx_batch {
+    int32_t abi_version;
+    int32_t nprob;
+    int32_t dtype, tab_fmt, dynamic, neg_trav;
+    int32_t rblock;            /* forward receivers per workgroup (all problems) */
+    int32_t sorder;            /* forward lane order (all problems) */
+    int32_t gsplit;            /* adjoint waves per tile (all problems) */
+    int32_t nt_max, nh_max;
+    int32_t pad0;
+    int64_t model_len;         /* elements of the concatenated models */
+    int64_t data_len;          /* elements of the concatenated data */
+    int64_t n_spread, n_gather, n_conv; /* work items */
+    int32_t grid_spread, grid_gather;   /* workgroups of the persistent launches */
+    void *dev;                 /* kx_batch_dev_bytes of device memory, caller-owned */
+} kx_batch;
+
+/* Device bytes a batch of these plans needs (descriptors, work lists,
+ * counters).  rblock, sorder and gsplit of plans[0] apply to the whole batch
+ * (0 = automatic), the other plans' values are ignored. */
+int64_t kx_batch_dev_bytes(const kx_plan *plans, int32_t nprob);
+/* Fills *b and writes the descriptors and work lists to dev; returns after
+ * the copies are done (not capturable).  The plans' tables and wavelets must
+ * outlive the batch; their aux buffers are not used.  Launches on one batch
+ * share its work counters: order them on one stream. */
+int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, void *dev,
+                   void *stream);
+/* Bytes of workspace of kx_batch_forward / kx_batch_adjoint (data_len
+ * elements of dtype). */
+int64_t kx_batch_workspace_bytes(const kx_batch *b);
+int kx_batch_forward(const kx_batch *b, const void *m, void *d, void *work, void *stream);
+int kx_batch_adjoint(const kx_batch *b, const void *d, void *m, void *work, void *stream);
+
+/* Segments of a concatenated vector, cut into chunks inside segments (all
+ * arrays on the device): segment p = [off[p], off[p+1]), chunk c =
+ * [chunk_lo[c], chunk_lo[c+1]) of segment chunk_seg[c], and the chunks of
+ * segment p are [seg_chunk[p], seg_chunk[p+1]). */
+typedef struct kx_segs {
+    int32_t nseg, nchunk;
+    const int64_t *off;
+    const int32_t *chunk_seg;
+    const int64_t *chunk_lo;
+    const int32_t *seg_chunk;
+} kx_segs;
+
+/* Bytes of reduction workspace for a segment set. */
+int64_t kx_seg_workspace_bytes(const kx_segs *s);
+/* Segment-wise kx_vec / kx_dot_dev: the scalars of segment p are
+ * sc[p * nsc + column]; coef, out and gate are columns (out, gate: -1 for
+ * none).  A segment whose gate scalar is 0 is left untouched. */
+int kx_seg_vec(int32_t op, const kx_segs *s, int32_t dtype, const void *x, void *y, double *sc,
+               int32_t nsc, int32_t coef, double sign, int32_t out, int32_t gate, void *work,
+               void *stream);
+int kx_seg_dot(const kx_segs *s, int32_t dtype, const void *a, const void *b, double *sc,
+               int32_t nsc, int32_t out, int32_t gate, void *work, void *stream);
+/* kx_cgls_scalar for nseg problems at once: scalars sc[p * KX_CG_NSC + ...],
+ * costs cost[p * ncost + iter], cost1 likewise. */
+int kx_cgls_scalar_batch(double *sc, int32_t nseg, double *cost, double *cost1, int32_t ncost,
+                         int32_t phase, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

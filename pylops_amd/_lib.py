@@ -23,7 +23,9 @@ EXPORTS = (
     "kx_tile_table", "kx_spread", "kx_gather", "kx_conv", "kx_forward",
     "kx_adjoint", "kx_reduce_workspace_bytes", "kx_dot",
     "kx_solver_workspace_bytes", "kx_vec", "kx_dot_dev", "kx_lsqr_xw",
-    "kx_cgls_scalar", "kx_lsqr_scalar",
+    "kx_cgls_scalar", "kx_lsqr_scalar", "kx_batch_dev_bytes", "kx_batch_build",
+    "kx_batch_workspace_bytes", "kx_batch_forward", "kx_batch_adjoint",
+    "kx_seg_workspace_bytes", "kx_seg_vec", "kx_seg_dot", "kx_cgls_scalar_batch",
 )
 
 # solver scalar layouts and vector ops (include/kirchx.h)
@@ -125,6 +127,44 @@ class KxPlan(ctypes.Structure):
     ]
 
 
+class KxBatch(ctypes.Structure):
+    """Mirror of ``kx_batch`` in include/kirchx.h."""
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("nprob", ctypes.c_int32),
+        ("dtype", ctypes.c_int32),
+        ("tab_fmt", ctypes.c_int32),
+        ("dynamic", ctypes.c_int32),
+        ("neg_trav", ctypes.c_int32),
+        ("rblock", ctypes.c_int32),
+        ("sorder", ctypes.c_int32),
+        ("gsplit", ctypes.c_int32),
+        ("nt_max", ctypes.c_int32),
+        ("nh_max", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("model_len", ctypes.c_int64),
+        ("data_len", ctypes.c_int64),
+        ("n_spread", ctypes.c_int64),
+        ("n_gather", ctypes.c_int64),
+        ("n_conv", ctypes.c_int64),
+        ("grid_spread", ctypes.c_int32),
+        ("grid_gather", ctypes.c_int32),
+        ("dev", ctypes.c_void_p),
+    ]
+
+
+class KxSegs(ctypes.Structure):
+    """Mirror of ``kx_segs`` in include/kirchx.h."""
+    _fields_ = [
+        ("nseg", ctypes.c_int32),
+        ("nchunk", ctypes.c_int32),
+        ("off", ctypes.c_void_p),
+        ("chunk_seg", ctypes.c_void_p),
+        ("chunk_lo", ctypes.c_void_p),
+        ("seg_chunk", ctypes.c_void_p),
+    ]
+
+
 class KirchxError(RuntimeError):
     pass
 
@@ -135,6 +175,8 @@ _lib = None
 def _declare(L):
     P, I32, I64, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     PP = ctypes.POINTER(KxPlan)
+    PB = ctypes.POINTER(KxBatch)
+    PS = ctypes.POINTER(KxSegs)
     sig = {
         "kx_version": ([], ctypes.c_int),
         "kx_last_error": ([], ctypes.c_char_p),
@@ -154,6 +196,15 @@ def _declare(L):
         "kx_lsqr_xw": ([I64, I32, P, P, P, P, P, P], ctypes.c_int),
         "kx_cgls_scalar": ([P, P, P, I32, P], ctypes.c_int),
         "kx_lsqr_scalar": ([P, P, I32, P], ctypes.c_int),
+        "kx_batch_dev_bytes": ([PP, I32], I64),
+        "kx_batch_build": ([PB, PP, I32, P, P], ctypes.c_int),
+        "kx_batch_workspace_bytes": ([PB], I64),
+        "kx_batch_forward": ([PB, P, P, P, P], ctypes.c_int),
+        "kx_batch_adjoint": ([PB, P, P, P, P], ctypes.c_int),
+        "kx_seg_workspace_bytes": ([PS], I64),
+        "kx_seg_vec": ([I32, PS, I32, P, P, P, I32, I32, D, I32, I32, P, P], ctypes.c_int),
+        "kx_seg_dot": ([PS, I32, P, P, P, I32, I32, I32, P, P], ctypes.c_int),
+        "kx_cgls_scalar_batch": ([P, I32, P, P, I32, I32, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

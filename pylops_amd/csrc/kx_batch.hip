@@ -1,0 +1,785 @@
+// kx_batch.hip -- batches of independent Kirchhoff problems (include/kirchx.h).
+//
+// Replaces the reference's one-problem-at-a-time use of Kirchhoff._matvec /
+// _rmatvec (pylops/waveeqprocessing/kirchhoff.py:1048-1143) when many
+// problems of uneven sizes are inverted together.  Every operator application
+// of a batch is one persistent launch per engine kernel: a grid sized to fill
+// the chip pulls work items -- (problem, source, receiver block) for the
+// spread, (problem, tile group) for the gather -- from one atomic counter, in
+// order of decreasing cost, so that small and large problems share the CUs
+// without a launch per problem.  The item bodies are the single-problem ones
+// (kx_device.h), so a problem's results do not depend on the batch it is in.
+//
+// Device memory (caller-owned, kx_batch_dev_bytes), 256-byte aligned parts:
+//   BatchProb   probs[nprob]       geometry, tables, offsets into m and d
+//   SpreadScale scales[nprob]      fixed-point scale of each problem's forward
+//   float       part[nprob][kBatchScaleBlocks]  partial max|m|
+//   unsigned    ctr[kCounters]     work counters (spread, gather), left zero
+//   SpreadItem  sitems[n_spread]   decreasing cost
+//   GatherItem  gitems[n_gather]   decreasing cost
+//   int64_t     cprefix[nprob + 1] convolution blocks before each problem
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "kirchx.h"
+
+extern "C" int kx_solver_fail(int code, const char *msg);
+extern "C" int kx_plan_check(const kx_plan *p);
+
+#include "kx_device.h"
+
+namespace {
+
+int bfail(int code, const char *fmt, ...)
+{
+    char buf[480];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    return kx_solver_fail(code, buf);
+}
+
+#define KB_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return bfail((int)e_, "%s: %s", #call, hipGetErrorString(e_));                 \
+    } while (0)
+
+#define KB_LAUNCHED(name)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess)                                                              \
+            return bfail((int)e_, "%s launch: %s", name, hipGetErrorString(e_));           \
+    } while (0)
+
+constexpr int kBatchScaleBlocks = 32;
+constexpr int kCounters = 16;  // [0, 1] spread next / done, [2, 3] gather
+constexpr int kGatherFill = 24;
+constexpr int kLdsPerCu = 160 * 1024;
+constexpr int kMaxWavesPerCu = 32;
+constexpr int kRblocks[] = {16, 8, 4, 2, 1};
+
+struct BatchProb {
+    KArgs a;
+    int64_t moff, doff;  // first element of the problem's model and data
+    int64_t ni;          // image points
+    const void *h, *hstar;
+    int nh;
+    int nseg;            // convolution blocks per trace
+    double wamp;
+};
+
+struct SpreadItem {
+    int p, s, r0, pad;
+};
+
+struct GatherItem {
+    int p, L;
+};
+
+// ---------------------------------------------------------------------------
+// persistent work loop
+// ---------------------------------------------------------------------------
+// Next item of the launch: thread 0 takes it from the counter, the block
+// reads it from LDS.  Returns the same value in every lane.
+__device__ __forceinline__ int next_item(unsigned *ctr, int *cur)
+{
+    if (threadIdx.x == 0)
+        *cur = (int)atomicAdd(ctr, 1u);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(*cur);
+    __syncthreads();  // read by every wave before thread 0 takes the next
+    return it;
+}
+
+// The last workgroup to finish resets the counters for the next launch: by
+// then every workgroup has taken its final item.
+__device__ __forceinline__ void launch_done(unsigned *ctr)
+{
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctr[1], 1u) == gridDim.x - 1) {
+            atomicExch(&ctr[0], 0u);
+            atomicExch(&ctr[1], 0u);
+        }
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+__global__ __launch_bounds__(kSpreadBlock) void batch_spread_kernel(
+    const BatchProb *__restrict__ probs, const SpreadScale *__restrict__ scales,
+    const SpreadItem *__restrict__ items, int nitems, unsigned *ctr, const TD *__restrict__ m,
+    TD *__restrict__ y)
+{
+    __shared__ int cur;
+    for (;;) {
+        const int it = next_item(ctr, &cur);
+        if (it >= nitems)
+            break;
+        const SpreadItem w = items[it];
+        const KArgs a = probs[w.p].a;
+        const SpreadScale sc = scales[w.p];
+        spread_item<TD, TT, DYN, NEG, ROT, RB>(a, m + probs[w.p].moff, y + probs[w.p].doff, sc,
+                                               w.s, w.r0);
+    }
+    launch_done(ctr);
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+__global__ __launch_bounds__(kGatherBlock) void batch_gather_kernel(
+    const BatchProb *__restrict__ probs, const GatherItem *__restrict__ items, int nitems,
+    unsigned *ctr, const TD *__restrict__ d, TD *__restrict__ m)
+{
+    __shared__ int cur;
+    for (;;) {
+        const int it = next_item(ctr, &cur);
+        if (it >= nitems)
+            break;
+        const GatherItem w = items[it];
+        const KArgs a = probs[w.p].a;
+        gather_tiles<TD, TT, DYN, NEG, JW, P>(a, d + probs[w.p].doff, m + probs[w.p].moff, w.L);
+    }
+    launch_done(ctr);
+}
+
+// One convolution block (kConvOut samples of one trace) per workgroup; the
+// problem is found by bisection of the block prefix.
+template <typename TD>
+__global__ __launch_bounds__(kBlock) void batch_conv_kernel(const BatchProb *__restrict__ probs,
+                                                            const int64_t *__restrict__ cprefix,
+                                                            int nprob, const TD *__restrict__ x,
+                                                            TD *__restrict__ y, int adjoint)
+{
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = nprob;  // cprefix[lo] <= b < cprefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cprefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const BatchProb &P = probs[lo];
+    const int64_t k = b - cprefix[lo];
+    const int nt = P.a.nt;
+    const TD *h = reinterpret_cast<const TD *>(adjoint ? P.hstar : P.h);
+    conv_segment<TD>(x + P.doff, y + P.doff, nt, h, P.nh, k / P.nseg,
+                     (int)(k % P.nseg) * kConvOut);
+}
+
+__global__ __launch_bounds__(kBlock) void batch_absmax_kernel(const BatchProb *__restrict__ probs,
+                                                              const float *__restrict__ m,
+                                                              float *__restrict__ part)
+{
+    const int p = blockIdx.y;
+    const float b =
+        absmax_block(m + probs[p].moff, probs[p].ni, blockIdx.x, kBatchScaleBlocks);
+    if (threadIdx.x == 0)
+        part[p * kBatchScaleBlocks + blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(kWave) void batch_scale_kernel(const BatchProb *__restrict__ probs,
+                                                            const float *__restrict__ part,
+                                                            SpreadScale *__restrict__ scales)
+{
+    const int p = blockIdx.x;
+    const float mx = absmax_wave(part + p * kBatchScaleBlocks, kBatchScaleBlocks);
+    if (threadIdx.x == 0)
+        scales[p] = scale_from_max(mx, probs[p].wamp);
+}
+
+// ---------------------------------------------------------------------------
+// host side: choices, layout, work lists
+// ---------------------------------------------------------------------------
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+int dsize(int dtype) { return dtype == KX_F64 ? 8 : 4; }
+
+struct Choice {
+    int rb, rot, P;
+};
+
+struct Layout {
+    size_t probs, scales, part, ctr, sitems, gitems, cprefix, total;
+    int64_t n_spread, n_gather, n_conv;
+};
+
+int device_cus()
+{
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    return cus;
+}
+
+// Largest receiver block per element type: registers per lane grow with the
+// block, and two workgroups per CU are kept.
+template <typename TD, bool DYN> constexpr int max_rblock()
+{
+    return sizeof(TD) == 8 ? 4 : (DYN ? 8 : 16);
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+const void *bspread_ptr()
+{
+    if constexpr (RB <= max_rblock<TD, DYN>())
+        return reinterpret_cast<const void *>(
+            &batch_spread_kernel<TD, TT, DYN, NEG, ROT && (RB > 1), RB>);
+    else
+        return nullptr;
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+const void *bspread_fn(int rb)
+{
+    switch (rb) {
+    case 16:
+        return bspread_ptr<TD, TT, DYN, NEG, ROT, 16>();
+    case 8:
+        return bspread_ptr<TD, TT, DYN, NEG, ROT, 8>();
+    case 4:
+        return bspread_ptr<TD, TT, DYN, NEG, ROT, 4>();
+    case 2:
+        return bspread_ptr<TD, TT, DYN, NEG, ROT, 2>();
+    default:
+        return bspread_ptr<TD, TT, DYN, NEG, ROT, 1>();
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, int JW>
+const void *bgather_fn(int P)
+{
+    switch (P) {
+    case 1:
+        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 1>);
+    case 2:
+        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 2>);
+    default:
+        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 4>);
+    }
+}
+
+// Resident workgroups of a kernel per CU; without a device, the LDS and
+// wave limits alone.
+int per_cu(const void *fn, int block, size_t lds)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, lds) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        n = kMaxWavesPerCu / (block / kWave);
+        if (lds > 0)
+            n = std::min(n, (int)(kLdsPerCu / (lds + 64)));
+    }
+    return std::max(n, 1);
+}
+
+size_t spread_lds(int rb, int nt) { return (size_t)rb * nt * 8; }
+
+// Kernel entry points of a batch, by its element and table types.
+struct Fns {
+    const void *(*spread)(int rb, bool rot, bool dyn, bool neg);
+    const void *(*gather)(int P, bool dyn, bool neg);
+};
+
+template <typename TD, typename TT>
+const void *spread_of(int rb, bool rot, bool dyn, bool neg)
+{
+    if (dyn)
+        return neg ? (rot ? bspread_fn<TD, TT, true, true, true>(rb)
+                          : bspread_fn<TD, TT, true, true, false>(rb))
+                   : (rot ? bspread_fn<TD, TT, true, false, true>(rb)
+                          : bspread_fn<TD, TT, true, false, false>(rb));
+    return neg ? (rot ? bspread_fn<TD, TT, false, true, true>(rb)
+                      : bspread_fn<TD, TT, false, true, false>(rb))
+               : (rot ? bspread_fn<TD, TT, false, false, true>(rb)
+                      : bspread_fn<TD, TT, false, false, false>(rb));
+}
+
+template <typename TD, typename TT>
+const void *gather_of(int P, bool dyn, bool neg)
+{
+    if (dyn)
+        return neg ? bgather_fn<TD, TT, true, true, 8>(P) : bgather_fn<TD, TT, true, false, 8>(P);
+    return neg ? bgather_fn<TD, TT, false, true, 4>(P) : bgather_fn<TD, TT, false, false, 4>(P);
+}
+
+Fns fns_of(int dtype, int tab_fmt)
+{
+    if (dtype == KX_F64)
+        return {spread_of<double, double>, gather_of<double, double>};
+    if (tab_fmt == KX_F16)
+        return {spread_of<float, __half>, gather_of<float, __half>};
+    return {spread_of<float, float>, gather_of<float, float>};
+}
+
+int check_batch(const kx_plan *plans, int nprob)
+{
+    if (!plans || nprob <= 0)
+        return bfail(KX_EINVAL, "batch: no plans");
+    const kx_plan &q = plans[0];
+    for (int p = 0; p < nprob; ++p) {
+        const kx_plan &a = plans[p];
+        int rc = kx_plan_check(&a);
+        if (rc)
+            return rc;
+        if (a.dtype != q.dtype || a.tab_fmt != q.tab_fmt || a.dynamic != q.dynamic ||
+            a.neg_trav != q.neg_trav)
+            return bfail(KX_EINVAL,
+                         "batch: problem %d differs from problem 0 in dtype, table format, "
+                         "dynamic or neg_trav", p);
+        if (!a.h || !a.hstar || a.nh <= 0 || (a.nh % 2) == 0)
+            return bfail(KX_EINVAL, "batch: problem %d has no odd-length wavelet", p);
+        if (spread_lds(1, a.nt) > (size_t)kLdsBudget)
+            return bfail(KX_ELDS, "batch: problem %d: nt=%d does not fit one trace in LDS", p,
+                         a.nt);
+        const size_t clds = (size_t)(a.nh + kConvOut + a.nh - 1) * dsize(a.dtype);
+        if (clds > (size_t)kLdsBudget)
+            return bfail(KX_ELDS, "batch: problem %d: filter of %d taps does not fit LDS", p,
+                         a.nh);
+    }
+    return KX_OK;
+}
+
+Choice choose(const kx_plan *plans, int nprob)
+{
+    const kx_plan &q = plans[0];
+    const Fns f = fns_of(q.dtype, q.tab_fmt);
+    int nt_max = 0, nr_max = 0;
+    int64_t tiles = 0;
+    for (int p = 0; p < nprob; ++p) {
+        nt_max = std::max(nt_max, plans[p].nt);
+        nr_max = std::max(nr_max, plans[p].nr);
+        tiles += plans[p].ntiles;
+    }
+    Choice c;
+    int order = q.sorder;
+    if (order == KX_ORDER_AUTO)
+        order = q.dynamic ? KX_ORDER_NATURAL : KX_ORDER_ROTATED;
+    c.rot = order == KX_ORDER_ROTATED;
+    // receivers per workgroup: as for one problem, on the batch's longest trace
+    const int max_rb = q.dtype == KX_F64 ? max_rblock<double, false>()
+                                         : (q.dynamic ? max_rblock<float, true>()
+                                                      : max_rblock<float, false>());
+    auto fits = [&](int rb) {
+        return rb <= max_rb && spread_lds(rb, nt_max) <= (size_t)kLdsBudget;
+    };
+    c.rb = 1;
+    if (q.rblock > 0) {
+        for (int rb : kRblocks)
+            if (rb <= q.rblock && fits(rb)) {
+                c.rb = rb;
+                break;
+            }
+    } else {
+        double best = -1.0;
+        for (int rb : kRblocks) {
+            if (!fits(rb) || (rb > 1 && rb / 2 >= nr_max))
+                continue;
+            const int n = per_cu(f.spread(rb, c.rot, q.dynamic, q.neg_trav), kSpreadBlock,
+                                 spread_lds(rb, nt_max));
+            const double waves = std::min(24.0, (double)n * kSpreadWaves);
+            const double score = waves / 24.0 * n * std::min(rb, nr_max);
+            if (score > best * 1.0001) {
+                best = score;
+                c.rb = rb;
+            }
+        }
+    }
+    if (c.rb == 1)
+        c.rot = 0;
+    // adjoint waves per tile: as for one problem, on the batch's tile count
+    c.P = 1;
+    if (q.gsplit > 0) {
+        while (c.P * 2 <= std::min(q.gsplit, kGatherWaves))
+            c.P *= 2;
+    } else {
+        const int cus = device_cus();
+        while (c.P < kGatherWaves && tiles * c.P < (int64_t)kGatherFill * cus)
+            c.P *= 2;
+    }
+    return c;
+}
+
+// Offsets of the parts of the device buffer.
+Layout place(int nprob, int64_t n_spread, int64_t n_gather, int64_t n_conv)
+{
+    Layout L;
+    L.n_spread = n_spread;
+    L.n_gather = n_gather;
+    L.n_conv = n_conv;
+    size_t o = 0;
+    L.probs = o;
+    o = align256(o + sizeof(BatchProb) * nprob);
+    L.scales = o;
+    o = align256(o + sizeof(SpreadScale) * nprob);
+    L.part = o;
+    o = align256(o + sizeof(float) * kBatchScaleBlocks * nprob);
+    L.ctr = o;
+    o = align256(o + sizeof(unsigned) * kCounters);
+    L.sitems = o;
+    o = align256(o + sizeof(SpreadItem) * n_spread);
+    L.gitems = o;
+    o = align256(o + sizeof(GatherItem) * n_gather);
+    L.cprefix = o;
+    o = align256(o + sizeof(int64_t) * (nprob + 1));
+    L.total = o;
+    return L;
+}
+
+Layout layout(const kx_plan *plans, int nprob, const Choice &c)
+{
+    int64_t ns = 0, ng = 0, nc = 0;
+    const int NT = kGatherWaves / c.P;
+    for (int p = 0; p < nprob; ++p) {
+        const kx_plan &a = plans[p];
+        ns += (int64_t)a.ns * ((a.nr + c.rb - 1) / c.rb);
+        ng += (a.ntiles + NT - 1) / NT;
+        nc += (int64_t)a.ns * a.nr * ((a.nt + kConvOut - 1) / kConvOut);
+    }
+    return place(nprob, ns, ng, nc);
+}
+
+struct View {
+    const BatchProb *probs;
+    SpreadScale *scales;
+    float *part;
+    unsigned *ctr;
+    const SpreadItem *sitems;
+    const GatherItem *gitems;
+    const int64_t *cprefix;
+};
+
+View view(const kx_batch *b)
+{
+    const Layout L = place(b->nprob, b->n_spread, b->n_gather, b->n_conv);
+    char *base = reinterpret_cast<char *>(b->dev);
+    View v;
+    v.probs = reinterpret_cast<const BatchProb *>(base + L.probs);
+    v.scales = reinterpret_cast<SpreadScale *>(base + L.scales);
+    v.part = reinterpret_cast<float *>(base + L.part);
+    v.ctr = reinterpret_cast<unsigned *>(base + L.ctr);
+    v.sitems = reinterpret_cast<const SpreadItem *>(base + L.sitems);
+    v.gitems = reinterpret_cast<const GatherItem *>(base + L.gitems);
+    v.cprefix = reinterpret_cast<const int64_t *>(base + L.cprefix);
+    return v;
+}
+
+int check_built(const kx_batch *b)
+{
+    if (!b || b->abi_version != KX_ABI_VERSION || b->nprob <= 0 || !b->dev)
+        return bfail(KX_EINVAL, "batch: not built");
+    return KX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+int launch_bspread(const kx_batch *b, const View &v, const void *m, void *y, hipStream_t st)
+{
+    if constexpr (RB <= max_rblock<TD, DYN>()) {
+        hipLaunchKernelGGL((batch_spread_kernel<TD, TT, DYN, NEG, ROT && (RB > 1), RB>),
+                           dim3(b->grid_spread), dim3(kSpreadBlock), spread_lds(RB, b->nt_max),
+                           st, v.probs, v.scales, v.sitems, (int)b->n_spread, v.ctr,
+                           reinterpret_cast<const TD *>(m), reinterpret_cast<TD *>(y));
+        KB_LAUNCHED("batch spread");
+        return KX_OK;
+    } else {
+        return bfail(KX_EINVAL, "batch: receiver block %d not supported for this type", RB);
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+int launch_bspread_rb(const kx_batch *b, const View &v, const void *m, void *y, hipStream_t st)
+{
+    switch (b->rblock) {
+    case 16:
+        return launch_bspread<TD, TT, DYN, NEG, ROT, 16>(b, v, m, y, st);
+    case 8:
+        return launch_bspread<TD, TT, DYN, NEG, ROT, 8>(b, v, m, y, st);
+    case 4:
+        return launch_bspread<TD, TT, DYN, NEG, ROT, 4>(b, v, m, y, st);
+    case 2:
+        return launch_bspread<TD, TT, DYN, NEG, ROT, 2>(b, v, m, y, st);
+    default:
+        return launch_bspread<TD, TT, DYN, NEG, ROT, 1>(b, v, m, y, st);
+    }
+}
+
+template <typename TD, typename TT>
+int launch_bspread_d(const kx_batch *b, const View &v, const void *m, void *y, hipStream_t st)
+{
+    const bool rot = b->sorder == KX_ORDER_ROTATED;
+    if (b->dynamic) {
+        if (b->neg_trav)
+            return rot ? launch_bspread_rb<TD, TT, true, true, true>(b, v, m, y, st)
+                       : launch_bspread_rb<TD, TT, true, true, false>(b, v, m, y, st);
+        return rot ? launch_bspread_rb<TD, TT, true, false, true>(b, v, m, y, st)
+                   : launch_bspread_rb<TD, TT, true, false, false>(b, v, m, y, st);
+    }
+    if (b->neg_trav)
+        return rot ? launch_bspread_rb<TD, TT, false, true, true>(b, v, m, y, st)
+                   : launch_bspread_rb<TD, TT, false, true, false>(b, v, m, y, st);
+    return rot ? launch_bspread_rb<TD, TT, false, false, true>(b, v, m, y, st)
+               : launch_bspread_rb<TD, TT, false, false, false>(b, v, m, y, st);
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+int launch_bgather(const kx_batch *b, const View &v, const void *d, void *m, hipStream_t st)
+{
+    hipLaunchKernelGGL((batch_gather_kernel<TD, TT, DYN, NEG, JW, P>), dim3(b->grid_gather),
+                       dim3(kGatherBlock), 0, st, v.probs, v.gitems, (int)b->n_gather, v.ctr + 2,
+                       reinterpret_cast<const TD *>(d), reinterpret_cast<TD *>(m));
+    KB_LAUNCHED("batch gather");
+    return KX_OK;
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG>
+int launch_bgather_p(const kx_batch *b, const View &v, const void *d, void *m, hipStream_t st)
+{
+    constexpr int JW = DYN ? 8 : 4;
+    switch (b->gsplit) {
+    case 1:
+        return launch_bgather<TD, TT, DYN, NEG, JW, 1>(b, v, d, m, st);
+    case 2:
+        return launch_bgather<TD, TT, DYN, NEG, JW, 2>(b, v, d, m, st);
+    default:
+        return launch_bgather<TD, TT, DYN, NEG, JW, 4>(b, v, d, m, st);
+    }
+}
+
+template <typename TD, typename TT>
+int launch_bgather_d(const kx_batch *b, const View &v, const void *d, void *m, hipStream_t st)
+{
+    if (b->dynamic)
+        return b->neg_trav ? launch_bgather_p<TD, TT, true, true>(b, v, d, m, st)
+                           : launch_bgather_p<TD, TT, true, false>(b, v, d, m, st);
+    return b->neg_trav ? launch_bgather_p<TD, TT, false, true>(b, v, d, m, st)
+                       : launch_bgather_p<TD, TT, false, false>(b, v, d, m, st);
+}
+
+int launch_bconv(const kx_batch *b, const View &v, const void *x, void *y, int adjoint,
+                 hipStream_t st)
+{
+    const size_t lds = (size_t)(b->nh_max + kConvOut + b->nh_max - 1) * dsize(b->dtype);
+    if (b->dtype == KX_F64)
+        hipLaunchKernelGGL(batch_conv_kernel<double>, dim3((unsigned)b->n_conv), dim3(kBlock), lds,
+                           st, v.probs, v.cprefix, b->nprob, reinterpret_cast<const double *>(x),
+                           reinterpret_cast<double *>(y), adjoint);
+    else
+        hipLaunchKernelGGL(batch_conv_kernel<float>, dim3((unsigned)b->n_conv), dim3(kBlock), lds,
+                           st, v.probs, v.cprefix, b->nprob, reinterpret_cast<const float *>(x),
+                           reinterpret_cast<float *>(y), adjoint);
+    KB_LAUNCHED("batch conv");
+    return KX_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int64_t kx_batch_dev_bytes(const kx_plan *plans, int32_t nprob)
+{
+    if (check_batch(plans, nprob))
+        return -1;
+    const Layout L = layout(plans, nprob, choose(plans, nprob));
+    return (int64_t)L.total;
+}
+
+extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, void *dev,
+                              void *stream)
+{
+    if (!b || !dev)
+        return bfail(KX_EINVAL, "batch: NULL batch or device buffer");
+    int rc = check_batch(plans, nprob);
+    if (rc)
+        return rc;
+    const Choice c = choose(plans, nprob);
+    const Layout L = layout(plans, nprob, c);
+    if (L.n_spread > INT32_MAX || L.n_gather > INT32_MAX || L.n_conv > INT32_MAX)
+        return bfail(KX_ESHAPE, "batch: more than 2^31 work items");
+    const kx_plan &q = plans[0];
+    const int JW = q.dynamic ? 8 : 4;
+    const int NT = kGatherWaves / c.P;
+
+    // descriptors
+    std::vector<BatchProb> probs(nprob);
+    std::vector<int64_t> cprefix(nprob + 1, 0);
+    int64_t moff = 0, doff = 0;
+    int nt_max = 0, nh_max = 0;
+    for (int p = 0; p < nprob; ++p) {
+        const kx_plan &a = plans[p];
+        if ((int64_t)JW * c.P * a.nr * a.nt >= ((int64_t)1 << 31))
+            return bfail(KX_ESHAPE, "batch: problem %d: %d sources x nr x nt exceeds 2^31 samples",
+                         p, JW * c.P);
+        BatchProb &B = probs[p];
+        B.a = make_args(&a);
+        B.a.rblock = c.rb;
+        B.moff = moff;
+        B.doff = doff;
+        B.ni = (int64_t)a.nx * a.ny * a.nz;
+        B.h = a.h;
+        B.hstar = a.hstar;
+        B.nh = a.nh;
+        B.nseg = (a.nt + kConvOut - 1) / kConvOut;
+        B.wamp = a.wamp;
+        moff += B.ni;
+        doff += (int64_t)a.ns * a.nr * a.nt;
+        cprefix[p + 1] = cprefix[p] + (int64_t)a.ns * a.nr * B.nseg;
+        nt_max = std::max(nt_max, a.nt);
+        nh_max = std::max(nh_max, a.nh);
+    }
+
+    // work lists in order of decreasing cost (a stable sort keeps a problem's
+    // items in their natural order)
+    std::vector<SpreadItem> sitems;
+    std::vector<int64_t> scost;
+    sitems.reserve(L.n_spread);
+    scost.reserve(L.n_spread);
+    for (int p = 0; p < nprob; ++p) {
+        const kx_plan &a = plans[p];
+        for (int s = 0; s < a.ns; ++s)
+            for (int r0 = 0; r0 < a.nr; r0 += c.rb) {
+                const int nrl = std::min(c.rb, a.nr - r0);
+                sitems.push_back({p, s, r0, 0});
+                // tile sweeps plus clearing and writing out the traces
+                scost.push_back(a.ntiles * kWave * nrl + (int64_t)nrl * a.nt * 4);
+            }
+    }
+    std::vector<int> order(sitems.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int i, int j) { return scost[i] > scost[j]; });
+    std::vector<SpreadItem> ssorted(sitems.size());
+    for (size_t i = 0; i < order.size(); ++i)
+        ssorted[i] = sitems[order[i]];
+
+    std::vector<GatherItem> gitems;
+    std::vector<int64_t> gcost;
+    gitems.reserve(L.n_gather);
+    gcost.reserve(L.n_gather);
+    for (int p = 0; p < nprob; ++p) {
+        const kx_plan &a = plans[p];
+        const int ng = (int)((a.ntiles + NT - 1) / NT);
+        for (int g = 0; g < ng; ++g) {
+            const int nti = std::min<int64_t>(NT, a.ntiles - (int64_t)g * NT);
+            gitems.push_back({p, g});
+            gcost.push_back((int64_t)nti * a.ns * a.nr);
+        }
+    }
+    order.resize(gitems.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int i, int j) { return gcost[i] > gcost[j]; });
+    std::vector<GatherItem> gsorted(gitems.size());
+    for (size_t i = 0; i < order.size(); ++i)
+        gsorted[i] = gitems[order[i]];
+
+    // persistent grids: the resident workgroups of the whole chip
+    const Fns f = fns_of(q.dtype, q.tab_fmt);
+    const int cus = device_cus();
+    const int sp_cu = per_cu(f.spread(c.rb, c.rot, q.dynamic, q.neg_trav), kSpreadBlock,
+                             spread_lds(c.rb, nt_max));
+    const int ga_cu = per_cu(f.gather(c.P, q.dynamic, q.neg_trav), kGatherBlock, 0);
+
+    b->abi_version = KX_ABI_VERSION;
+    b->nprob = nprob;
+    b->dtype = q.dtype;
+    b->tab_fmt = q.tab_fmt;
+    b->dynamic = q.dynamic;
+    b->neg_trav = q.neg_trav;
+    b->rblock = c.rb;
+    b->sorder = c.rot ? KX_ORDER_ROTATED : KX_ORDER_NATURAL;
+    b->gsplit = c.P;
+    b->nt_max = nt_max;
+    b->nh_max = nh_max;
+    b->pad0 = 0;
+    b->model_len = moff;
+    b->data_len = doff;
+    b->n_spread = L.n_spread;
+    b->n_gather = L.n_gather;
+    b->n_conv = L.n_conv;
+    b->grid_spread = (int)std::min<int64_t>(L.n_spread, (int64_t)cus * sp_cu);
+    b->grid_gather = (int)std::min<int64_t>(L.n_gather, (int64_t)cus * ga_cu);
+    b->dev = dev;
+
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char *base = reinterpret_cast<char *>(dev);
+    KB_HIP(hipMemsetAsync(base, 0, L.total, st));
+    KB_HIP(hipMemcpyAsync(base + L.probs, probs.data(), sizeof(BatchProb) * nprob,
+                          hipMemcpyHostToDevice, st));
+    KB_HIP(hipMemcpyAsync(base + L.sitems, ssorted.data(), sizeof(SpreadItem) * ssorted.size(),
+                          hipMemcpyHostToDevice, st));
+    KB_HIP(hipMemcpyAsync(base + L.gitems, gsorted.data(), sizeof(GatherItem) * gsorted.size(),
+                          hipMemcpyHostToDevice, st));
+    KB_HIP(hipMemcpyAsync(base + L.cprefix, cprefix.data(), sizeof(int64_t) * (nprob + 1),
+                          hipMemcpyHostToDevice, st));
+    KB_HIP(hipStreamSynchronize(st));  // the host vectors end here
+    return KX_OK;
+}
+
+extern "C" int64_t kx_batch_workspace_bytes(const kx_batch *b)
+{
+    if (check_built(b))
+        return -1;
+    return b->data_len * dsize(b->dtype);
+}
+
+extern "C" int kx_batch_forward(const kx_batch *b, const void *m, void *d, void *work, void *stream)
+{
+    int rc = check_built(b);
+    if (rc)
+        return rc;
+    if (!m || !d || !work)
+        return bfail(KX_EINVAL, "batch forward: NULL buffer");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const View v = view(b);
+    if (b->dtype == KX_F32) {
+        hipLaunchKernelGGL(batch_absmax_kernel, dim3(kBatchScaleBlocks, b->nprob), dim3(kBlock), 0,
+                           st, v.probs, reinterpret_cast<const float *>(m), v.part);
+        KB_LAUNCHED("batch scale");
+        hipLaunchKernelGGL(batch_scale_kernel, dim3(b->nprob), dim3(kWave), 0, st, v.probs, v.part,
+                           v.scales);
+        KB_LAUNCHED("batch scale");
+    }
+    if (b->dtype == KX_F64)
+        rc = launch_bspread_d<double, double>(b, v, m, work, st);
+    else if (b->tab_fmt == KX_F16)
+        rc = launch_bspread_d<float, __half>(b, v, m, work, st);
+    else
+        rc = launch_bspread_d<float, float>(b, v, m, work, st);
+    if (rc)
+        return rc;
+    return launch_bconv(b, v, work, d, 0, st);
+}
+
+extern "C" int kx_batch_adjoint(const kx_batch *b, const void *d, void *m, void *work, void *stream)
+{
+    int rc = check_built(b);
+    if (rc)
+        return rc;
+    if (!m || !d || !work)
+        return bfail(KX_EINVAL, "batch adjoint: NULL buffer");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const View v = view(b);
+    rc = launch_bconv(b, v, d, work, 1, st);
+    if (rc)
+        return rc;
+    if (b->dtype == KX_F64)
+        return launch_bgather_d<double, double>(b, v, work, m, st);
+    if (b->tab_fmt == KX_F16)
+        return launch_bgather_d<float, __half>(b, v, work, m, st);
+    return launch_bgather_d<float, float>(b, v, work, m, st);
+}

@@ -1,0 +1,634 @@
+// kx_device.h -- device code shared by the single-problem (kirchx.hip) and
+// batched (kx_batch.hip) launches of the Kirchhoff kernels: data types, tile
+// geometry and the bodies of the forward, adjoint and convolution kernels,
+// each taking its work item explicitly.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kirchx.h"
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kLdsBudget = 64 * 1024;
+constexpr int kReduceBlocks = 1024;
+
+template <typename TD> struct TimeType { using type = float; };
+template <> struct TimeType<double> { using type = double; };
+
+template <typename T, int N> struct alignas(sizeof(T) * N) VecN {
+    T v[N];
+};
+
+__device__ __forceinline__ float ld_off(const float *p) { return *p; }
+__device__ __forceinline__ float ld_off(const __half *p) { return __half2float(*p); }
+__device__ __forceinline__ double ld_off(const double *p) { return *p; }
+
+// Hardware dispatches workgroup b to XCD (b mod 8).  Renumber so that each
+// XCD receives a contiguous range of logical blocks: neighbouring image
+// tiles (gather) or the same receiver block (spread) then share one L2.
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    const int k = b & 7;
+    const int q = n >> 3;
+    const int rem = n & 7;
+    return k * q + (k < rem ? k : rem) + (b >> 3);
+}
+
+struct KArgs {
+    int ns, nr, nt;
+    int nx, ny, nz;
+    int tz, tx, ty;
+    int ntz, ntx, nty;
+    int ntiles;
+    int rblock;
+    const void *off_s;
+    const int *base_s;
+    const void *off_r;
+    const int *base_r;
+    const void *wp_s, *wq_s, *wp_r, *wq_r;
+    const int *izmin;
+};
+
+inline KArgs make_args(const kx_plan *p)
+{
+    KArgs a;
+    a.ns = p->ns;
+    a.nr = p->nr;
+    a.nt = p->nt;
+    a.nx = p->nx;
+    a.ny = p->ny;
+    a.nz = p->nz;
+    a.tz = p->tz;
+    a.tx = p->tx;
+    a.ty = p->ty;
+    a.ntz = p->ntz;
+    a.ntx = p->ntx;
+    a.nty = p->nty;
+    a.ntiles = (int)p->ntiles;
+    a.rblock = p->rblock;
+    a.off_s = p->off_s;
+    a.base_s = p->base_s;
+    a.off_r = p->off_r;
+    a.base_r = p->base_r;
+    a.wp_s = p->wp_s;
+    a.wq_s = p->wq_s;
+    a.wp_r = p->wp_r;
+    a.wq_r = p->wq_r;
+    a.izmin = p->izmin;
+    return a;
+}
+
+struct TilePoint {
+    int ix, iy, iz;
+    bool valid;
+};
+
+__device__ __forceinline__ TilePoint tile_point(const KArgs &a, int t, int lane)
+{
+    const int lz = lane % a.tz;
+    const int lx = (lane / a.tz) % a.tx;
+    const int ly = lane / (a.tz * a.tx);
+    const int tzi = t % a.ntz;
+    const int rest = t / a.ntz;
+    const int txi = rest % a.ntx;
+    const int tyi = rest / a.ntx;
+    TilePoint p;
+    p.iz = tzi * a.tz + lz;
+    p.ix = txi * a.tx + lx;
+    p.iy = tyi * a.ty + ly;
+    p.valid = (p.iz < a.nz) & (p.ix < a.nx) & (p.iy < a.ny);
+    return p;
+}
+
+__device__ __forceinline__ int64_t point_index(const KArgs &a, const TilePoint &p)
+{
+    return ((int64_t)p.iy * a.nx + p.ix) * a.nz + p.iz;
+}
+
+// ---------------------------------------------------------------------------
+// forward: spread into LDS-resident traces
+// ---------------------------------------------------------------------------
+// LDS accumulation.  For float data every contribution c (rounded to float
+// as the reference rounds it) is added as the integer rint(c * 2^e) to a
+// 64-bit fixed-point sample, with one power-of-two scale 2^e per launch
+// chosen from max|m| so that |c| 2^e <= 2^30: the rounding of a contribution
+// is at most 2^-31 of the largest possible one, the sum itself is exact and
+// does not depend on the order of the adds, and it cannot overflow (at most
+// 2^32 contributions per sample).  On gfx950 LDS integer atomics sustain
+// ~1.5x the rate of float64 ones and float32 LDS atomics ~1/6 of float64.
+// A launch whose m holds inf or NaN accumulates in float64 instead, so that
+// they propagate as in the reference; double data always does.
+constexpr int kSpreadWaves = 8;
+constexpr int kSpreadBlock = kSpreadWaves * kWave;
+constexpr int kScaleBlocks = 256;
+
+// aux layout: [0] int exponent e, [1] int 1 = fixed point / 0 = float64,
+// [16, 16 + kScaleBlocks) float partial maxima
+struct SpreadScale {
+    int e;
+    int fixed;
+};
+
+// Running tile coordinates of a wave: tiles are visited t, t + step, ...;
+// advancing carries through z, then x, then y without a division.
+struct TileCursor {
+    int t, tzi, txi, tyi;
+    __device__ __forceinline__ void start(const KArgs &a, int t0)
+    {
+        t = t0;
+        tzi = t0 % a.ntz;
+        const int rest = t0 / a.ntz;
+        txi = rest % a.ntx;
+        tyi = rest / a.ntx;
+    }
+    __device__ __forceinline__ void advance(const KArgs &a, int step)
+    {
+        t += step;
+        tzi += step;
+        while (tzi >= a.ntz) {
+            tzi -= a.ntz;
+            if (++txi == a.ntx) {
+                txi = 0;
+                ++tyi;
+            }
+        }
+    }
+};
+
+// Opaque zero held in a VGPR: indexing with it keeps a wave-uniform load on
+// the vector memory path.  Scalar loads share the lgkm counter with LDS
+// operations, so waiting on a scalar load would also wait for every LDS
+// atomic still in flight.
+__device__ __forceinline__ int vgpr_zero()
+{
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    return z;
+}
+
+// Lane roles.  Lane p holds point p of the tile and reads the source side
+// of its point once per tile.  At step r (r < RB) it treats receiver j of
+// the workgroup's block: j = r in natural order, so that the table loads of
+// a step are coalesced rows, or j = (p + r) mod RB in rotated order, so that
+// one atomic instruction covers all RB trace rows at 64 / RB points RB apart
+// each.  Samples of neighbouring points of one trace often coincide, and the
+// lanes of an LDS atomic that hit one address are serialised: the rotated
+// order trades scattered table loads for fewer such collisions.
+template <int RB, bool ROT> __device__ __forceinline__ int step_receiver(int lane, int r)
+{
+    return ROT ? (lane + r) & (RB - 1) : r;
+}
+
+// One lane's share of one tile: the image value and the table entries of
+// its point, receivers in the lane's step order.
+template <typename TD, typename TT, int RB> struct SpreadTile {
+    TD mv;
+    TT os;       // raw table elements: converted at use, so that a prefetch
+    TT orr[RB];  // never waits for them
+    TD ps, qs;
+    TD pr[RB], qr[RB];
+    int bs;
+    int br[RB];
+};
+
+template <typename TD, typename TT, bool DYN, bool ROT, int RB>
+__device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c, int lane, int rel,
+                                            int lz, int lx, int ly, int s, int r0, int nrl,
+                                            const TD *__restrict__ m, int vz,
+                                            SpreadTile<TD, TT, RB> &L)
+{
+    const int t = c.t;
+    const int iz = c.tzi * a.tz + lz, ix = c.txi * a.tx + lx, iy = c.tyi * a.ty + ly;
+    const bool valid = (iz < a.nz) & (ix < a.nx) & (iy < a.ny);
+    const int orig = (c.tyi * a.ty * a.nx + c.txi * a.tx) * a.nz + c.tzi * a.tz;
+    L.mv = valid ? m[orig + rel] : TD(0);
+    const size_t ts = (size_t)t * a.ns + s;        // (tile, source) row
+    const size_t tr = (size_t)t * a.nr + r0;       // (tile, first receiver) row
+    L.os = reinterpret_cast<const TT *>(a.off_s)[ts * kWave + lane];
+    L.bs = a.base_s[ts + vz];
+    if constexpr (DYN) {
+        L.ps = reinterpret_cast<const TD *>(a.wp_s)[ts * kWave + lane];
+        L.qs = reinterpret_cast<const TD *>(a.wq_s)[ts * kWave + lane];
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        int j = step_receiver<RB, ROT>(lane, r);
+        j = j < nrl ? j : 0;  // rows past the last receiver read row 0
+        const size_t e = (tr + j) * kWave + lane;
+        L.orr[r] = reinterpret_cast<const TT *>(a.off_r)[e];
+        L.br[r] = a.base_r[tr + j];
+        if constexpr (DYN) {
+            L.pr[r] = reinterpret_cast<const TD *>(a.wp_r)[e];
+            L.qr[r] = reinterpret_cast<const TD *>(a.wq_r)[e];
+        }
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
+__device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
+                                            TD *__restrict__ y, int e, int s, int r0)
+{
+    using TC = typename TimeType<TD>::type;
+    extern __shared__ __align__(16) unsigned char kx_smem[];
+    double *accd = reinterpret_cast<double *>(kx_smem);
+    unsigned long long *acci = reinterpret_cast<unsigned long long *>(kx_smem);
+
+    // one workgroup = one source s and receivers [r0, r0 + RB)
+    const int nrl = min(RB, a.nr - r0);
+    const int ntr = nrl * a.nt;
+
+    for (int i = threadIdx.x; i < ntr; i += blockDim.x)
+        acci[i] = 0ull;  // 0.0 and integer 0 share the bit pattern
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nw = blockDim.x / kWave;
+    const int vz = vgpr_zero();
+    const int lz = lane % a.tz, lx = (lane / a.tz) % a.tx, ly = lane / (a.tz * a.tx);
+    const int rel = (ly * a.nx + lx) * a.nz + lz;  // image offset from the tile origin
+    int lrow[RB];                                  // LDS row of the receiver of step r
+    int izm[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int j = step_receiver<RB, ROT>(lane, r);
+        lrow[r] = j < nrl ? j * a.nt : -1;
+        if constexpr (DYN)
+            izm[r] = a.izmin[s * a.nr + r0 + (j < nrl ? j : 0)];
+    }
+    const unsigned ntm1 = (unsigned)(a.nt - 1);
+
+    SpreadTile<TD, TT, RB> cur, nxt;
+    TileCursor c;
+    c.start(a, wave);
+    if (c.t < a.ntiles)
+        spread_load<TD, TT, DYN, ROT, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, cur);
+    while (c.t < a.ntiles) {
+        // prefetch the next tile while this one is accumulated
+        TileCursor cn = c;
+        cn.advance(a, nw);
+        if (cn.t < a.ntiles)
+            spread_load<TD, TT, DYN, ROT, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
+        if (__ballot(cur.mv != TD(0)) != 0) {
+            const int iz = c.tzi * a.tz + lz;
+            const TC os = (TC)ld_off(&cur.os);
+            // the image value, scaled by 2^e for the fixed-point sum
+            TD mv = cur.mv;
+            if constexpr (FIX)
+                mv = ldexpf(mv, e);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                const TC T = os + (TC)ld_off(&cur.orr[r]);
+                const TC fl = floor(T);
+                TC f = T - fl;
+                int k = (int)fl + cur.bs + cur.br[r];
+                if constexpr (NEG) {
+                    // the reference truncates toward zero: a time in (-1, 0)
+                    // lands on sample 0 with a negative weight
+                    if (k == -1 && f > TC(0)) {
+                        k = 0;
+                        f -= TC(1);
+                    }
+                }
+                TD w = mv;
+                if constexpr (DYN) {
+                    w = w * (cur.ps * cur.pr[r] + cur.qs * cur.qr[r]);
+                    if (iz < izm[r])
+                        w = TD(0);
+                }
+                if (lrow[r] >= 0 && (unsigned)k < ntm1 && w != TD(0)) {
+                    const int it = lrow[r] + k;
+                    const TD w0 = w * (TD)(TC(1) - f);
+                    const TD w1 = w * (TD)f;
+                    if constexpr (FIX) {
+                        atomicAdd(&acci[it], (unsigned long long)(long long)__float2int_rn(w0));
+                        atomicAdd(&acci[it + 1], (unsigned long long)(long long)__float2int_rn(w1));
+                    } else {
+                        atomicAdd(&accd[it], (double)w0);
+                        atomicAdd(&accd[it + 1], (double)w1);
+                    }
+                }
+            }
+        }
+        cur = nxt;
+        c = cn;
+    }
+    __syncthreads();
+    TD *yo = y + ((size_t)s * a.nr + r0) * a.nt;
+    for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
+        if constexpr (FIX)
+            yo[i] = (TD)ldexp((double)(long long)acci[i], -e);
+        else
+            yo[i] = (TD)accd[i];
+    }
+    __syncthreads();  // the traces are read out before a next item clears them
+}
+
+// One (source, receiver block) item with the launch's accumulation scale.
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+__device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict__ m,
+                                            TD *__restrict__ y, const SpreadScale &sc, int s,
+                                            int r0)
+{
+    if constexpr (sizeof(TD) == 4) {
+        if (sc.fixed)
+            spread_body<TD, TT, DYN, NEG, ROT, true, RB>(a, m, y, sc.e, s, r0);
+        else
+            spread_body<TD, TT, DYN, NEG, ROT, false, RB>(a, m, y, 0, s, r0);
+    } else {
+        spread_body<TD, TT, DYN, NEG, ROT, false, RB>(a, m, y, 0, s, r0);
+    }
+}
+
+// Per-launch scale of the fixed-point accumulation: partial max|m| per block
+// (inf for a non-finite value), then one wave turns the maximum into e.
+// Block b of nb strides over m[0, n); the result is valid in thread 0.
+__device__ __forceinline__ float absmax_block(const float *__restrict__ m, int64_t n, int b, int nb)
+{
+    float mx = 0.f;
+    const int64_t stride = (int64_t)nb * blockDim.x;
+    for (int64_t i = (int64_t)b * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float v = fabsf(m[i]);
+        if (!(v <= 3.402823466e38f))
+            v = INFINITY;  // NaN or inf
+        mx = fmaxf(mx, v);
+    }
+    __shared__ float wmax[kBlock / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        wmax[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    float r = wmax[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
+        r = fmaxf(r, wmax[i]);
+    return r;
+}
+
+// Maximum of np partial maxima, by one wave; valid in every lane.
+__device__ __forceinline__ float absmax_wave(const float *__restrict__ part, int np)
+{
+    float mx = 0.f;
+    for (int i = threadIdx.x; i < np; i += kWave)
+        mx = fmaxf(mx, part[i]);
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+    return mx;
+}
+
+// A contribution is at most |m| * wamp * 2 (the truncation path of a
+// negative time weighs a sample by up to 2): e puts that bound below 2^30.
+__device__ __forceinline__ SpreadScale scale_from_max(float mx, double wamp)
+{
+    const double bound = 2.0 * (double)mx * wamp;
+    SpreadScale sc;
+    if (!(bound <= 1.7976931348623157e308)) {
+        sc.e = 0;
+        sc.fixed = 0;
+    } else if (bound == 0.0) {
+        sc.e = 0;
+        sc.fixed = 1;
+    } else {
+        int ex;
+        frexp(bound, &ex);  // bound < 2^ex
+        sc.e = 30 - ex;
+        sc.fixed = 1;
+    }
+    return sc;
+}
+
+__global__ __launch_bounds__(kBlock) void absmax_partial_kernel(const float *__restrict__ m,
+                                                                int64_t n,
+                                                                float *__restrict__ part)
+{
+    const float b = absmax_block(m, n, blockIdx.x, gridDim.x);
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(kWave) void scale_final_kernel(const float *__restrict__ part, int np,
+                                                            double wamp,
+                                                            SpreadScale *__restrict__ sc)
+{
+    const float mx = absmax_wave(part, np);
+    if (threadIdx.x == 0)
+        *sc = scale_from_max(mx, wamp);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint: gather per image point from wave-staged trace windows
+// ---------------------------------------------------------------------------
+// A wave owns one tile and a share of the sources (P waves of a workgroup
+// split a tile's sources; their partial sums are added in a fixed order at
+// the end).  It sweeps the receivers r; for each of its JW sources the trace
+// window of kWin samples centred on the tile's base sum (source plus
+// receiver base: the tile's own offsets span a few tens of samples) is read
+// with one coalesced load -- issued a round ahead -- into the wave's own LDS
+// region, and each lane reads its two samples there.  A 64-lane gather from
+// global memory costs the texture path ~10x the coalesced load and LDS reads
+// that replace it.  Samples outside the window are read from global memory,
+// so the window size affects speed only.  No barrier until the final sum.
+constexpr int kGatherWaves = 4;
+constexpr int kGatherBlock = kGatherWaves * kWave;
+constexpr int kWin = kWave;  // samples per staged window, one per lane
+
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+__device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restrict__ d,
+                                             TD *__restrict__ m, int L)
+{
+    using TC = typename TimeType<TD>::type;
+    constexpr int NT = kGatherWaves / P;  // tiles per workgroup
+    constexpr int SB = JW * P;            // sources per block (all parts)
+    __shared__ TD win[kGatherWaves][JW][kWin];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int part = wave / NT;  // this wave sums sources s0 + part + P i
+    const int tt = L * NT + wave % NT;
+    const bool active = tt < a.ntiles;
+    const int t = active ? tt : a.ntiles - 1;
+    const TilePoint p = tile_point(a, t, lane);
+    const TT *offs = reinterpret_cast<const TT *>(a.off_s);
+    const TT *offr = reinterpret_cast<const TT *>(a.off_r);
+    const TD *wps = reinterpret_cast<const TD *>(a.wp_s);
+    const TD *wqs = reinterpret_cast<const TD *>(a.wq_s);
+    const TD *wpr = reinterpret_cast<const TD *>(a.wp_r);
+    const TD *wqr = reinterpret_cast<const TD *>(a.wq_r);
+    const unsigned ntm1 = (unsigned)(a.nt - 1);
+    const int vz = vgpr_zero();
+    TD(&w)[JW][kWin] = win[wave];
+
+    TD acc = TD(0);
+    if (active) {
+        for (int s0 = 0; s0 < a.ns; s0 += SB) {
+            // this wave's sources j = s0 + part + P i
+            TC os[JW];
+            int bs[JW];
+            TD ps[JW], qs[JW];
+            // trace (j, receiver 0) as an element offset from the block's
+            // first trace (fits 31 bits: checked at launch), held in vector
+            // registers: JW 64-bit pointers would exhaust the scalar ones
+            const TD *dblk = d + (size_t)s0 * a.nr * a.nt;
+            int soff[JW], izoff[JW];
+#pragma unroll
+            for (int i = 0; i < JW; ++i) {
+                const int j = s0 + part + P * i;
+                if (j < a.ns) {
+                    const size_t so = (size_t)t * a.ns + j;
+                    os[i] = (TC)ld_off(offs + so * kWave + lane);
+                    bs[i] = a.base_s[so + vz];
+                    if constexpr (DYN) {
+                        ps[i] = wps[so * kWave + lane];
+                        qs[i] = wqs[so * kWave + lane];
+                    }
+                } else {  // past the last source: every sample before the trace
+                    os[i] = TC(0);
+                    bs[i] = -(1 << 30);
+                    ps[i] = qs[i] = TD(0);
+                }
+                soff[i] = (min(j, a.ns - 1) - s0) * a.nr * a.nt + vz;
+                izoff[i] = min(j, a.ns - 1) * a.nr + vz;
+            }
+
+            // the window of source i in round r starts at bs + br(r) - kWin / 2;
+            // lane l stages its sample l
+            TD sv[JW];
+#define KX_STAGE(R, BR)                                                                        \
+    _Pragma("unroll") for (int i = 0; i < JW; ++i)                                              \
+    {                                                                                          \
+        const int e = bs[i] + (BR) - kWin / 2 + lane;                                          \
+        sv[i] = (unsigned)e < (unsigned)a.nt ? dblk[soff[i] + (R) * a.nt + e] : TD(0);          \
+    }
+            // receiver side of round r, and receiver base of round r + 1
+            TT orr_c = offr[((size_t)t * a.nr) * kWave + lane];
+            int br_c = a.base_r[(size_t)t * a.nr + vz];
+            TD pr_c = TD(0), qr_c = TD(0);
+            if constexpr (DYN) {
+                pr_c = wpr[((size_t)t * a.nr) * kWave + lane];
+                qr_c = wqr[((size_t)t * a.nr) * kWave + lane];
+            }
+            KX_STAGE(0, br_c)
+            for (int r = 0; r < a.nr; ++r) {
+                // this round's samples into the wave's LDS region
+#pragma unroll
+                for (int i = 0; i < JW; ++i)
+                    w[i][lane] = sv[i];
+                const TC orr = (TC)ld_off(&orr_c);
+                const int br = br_c;
+                const TD pr = pr_c, qr = qr_c;
+                // next round's receiver side and samples, in flight meanwhile
+                if (r + 1 < a.nr) {
+                    const size_t ro = (size_t)t * a.nr + r + 1;
+                    orr_c = offr[ro * kWave + lane];
+                    br_c = a.base_r[ro + vz];
+                    if constexpr (DYN) {
+                        pr_c = wpr[ro * kWave + lane];
+                        qr_c = wqr[ro * kWave + lane];
+                    }
+                    KX_STAGE(r + 1, br_c)
+                }
+#pragma unroll
+                for (int i = 0; i < JW; ++i) {
+                    const TC T = os[i] + orr;
+                    const TC fl = floor(T);
+                    TC f = T - fl;
+                    int it = (int)fl + bs[i] + br;
+                    if constexpr (NEG) {
+                        // the reference truncates toward zero: a time in
+                        // (-1, 0) reads sample 0 with a negative weight
+                        if (it == -1 && f > TC(0)) {
+                            it = 0;
+                            f -= TC(1);
+                        }
+                    }
+                    // The LDS read is unconditional (clamped index); a sample
+                    // outside the window overwrites it from global memory (a
+                    // branch between the two loads would merge them into one
+                    // generic-address load).
+                    const int kk = it - (bs[i] + br - kWin / 2);
+                    const int kc = min(max(kk, 0), kWin - 2);
+                    TD q0 = w[i][kc];
+                    TD q1 = w[i][kc + 1];
+                    const bool in_trace = (unsigned)it < ntm1;
+                    if (in_trace && kk != kc) {
+                        const TD *q = dblk + (soff[i] + r * a.nt + it);
+                        q0 = q[0];
+                        q1 = q[1];
+                    }
+                    TD v = q0 * (TD)(TC(1) - f) + q1 * (TD)f;
+                    if constexpr (DYN) {
+                        TD wgt = ps[i] * pr + qs[i] * qr;
+                        if (p.iz < a.izmin[izoff[i] + r])
+                            wgt = TD(0);
+                        v *= wgt;
+                    }
+                    acc += in_trace ? v : TD(0);
+                }
+            }
+#undef KX_STAGE
+        }
+    }
+    if constexpr (P > 1) {
+        // the partial sums of a tile's points, added in part order
+        __syncthreads();
+        TD *red = &win[0][0][0];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (part == 0) {
+            acc = TD(0);
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+                acc += red[(q * NT + wave) * kWave + lane];
+        }
+    }
+    if (part == 0 && active && p.valid)
+        m[point_index(a, p)] = acc;
+    if constexpr (P > 1)
+        __syncthreads();  // the partial sums are read before a next item stages
+}
+
+// ---------------------------------------------------------------------------
+// "same" convolution along time, odd filter
+// ---------------------------------------------------------------------------
+constexpr int kConvOut = 256;
+
+// Output samples [n0, n0 + kConvOut) of trace tr.
+template <typename TD>
+__device__ __forceinline__ void conv_segment(const TD *__restrict__ x, TD *__restrict__ y, int nt,
+                                             const TD *__restrict__ h, int nh, int64_t tr, int n0)
+{
+    extern __shared__ __align__(16) unsigned char kx_smem[];
+    TD *hs = reinterpret_cast<TD *>(kx_smem);
+    TD *win = hs + nh;
+    const int c = (nh - 1) / 2;
+    const int start = n0 + c - (nh - 1);
+    const int nwin = kConvOut + nh - 1;
+    const TD *xr = x + tr * nt;
+    for (int k = threadIdx.x; k < nh; k += blockDim.x)
+        hs[k] = h[nh - 1 - k];  // flipped: out[i] = sum_k hs[k] win[i + k]
+    for (int j = threadIdx.x; j < nwin; j += blockDim.x) {
+        const int n = start + j;
+        win[j] = (n >= 0 && n < nt) ? xr[n] : TD(0);
+    }
+    __syncthreads();
+    const int i = threadIdx.x;
+    if (i < kConvOut && n0 + i < nt) {
+        TD acc = TD(0);
+        for (int k = 0; k < nh; ++k)
+            acc += hs[k] * win[i + k];
+        y[tr * nt + n0 + i] = acc;
+    }
+    __syncthreads();  // the window is read before a next item loads its own
+}
+
+}  // namespace

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void lsqr_xw_kernel(int64_t n, T *__restric
 __device__ __forceinline__ double norm2(double a, double b) { return sqrt(a * a + b * b); }
 
 // CGLS scalar recurrences (CB:518-563)
-__global__ void cgls_scalar_kernel(double *sc, double *cost, double *cost1, int phase)
+__device__ __forceinline__ void cgls_scalars(double *sc, double *cost, double *cost1, int phase)
 {
     if (sc[KX_CG_ACTIVE] == 0.0)
         return;
@@ -187,6 +187,96 @@ __global__ void cgls_scalar_kernel(double *sc, double *cost, double *cost1, int 
         cost1[k] = sqrt(cs * cs + sc[KX_CG_DAMP2] * fabs(sc[KX_CG_XX]));
         sc[KX_CG_ACTIVE] = (it < sc[KX_CG_NITER] && sc[KX_CG_KOLD] > sc[KX_CG_TOL]) ? 1.0 : 0.0;
     }
+}
+
+__global__ void cgls_scalar_kernel(double *sc, double *cost, double *cost1, int phase)
+{
+    cgls_scalars(sc, cost, cost1, phase);
+}
+
+// one thread per problem of a batch
+__global__ void cgls_scalar_batch_kernel(double *sc, int nseg, double *cost, double *cost1,
+                                         int ncost, int phase)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nseg)
+        cgls_scalars(sc + (size_t)p * KX_CG_NSC, cost + (size_t)p * ncost,
+                     cost1 + (size_t)p * ncost, phase);
+}
+
+// ---------------------------------------------------------------------------
+// segment-wise vector algebra: one block per chunk, a chunk inside one
+// segment; partial sums per chunk, then one block per segment adds them
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool seg_on(const double *sc, int nsc, int p, int gate)
+{
+    return gate < 0 || sc[(size_t)p * nsc + gate] != 0.0;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void seg_vec_kernel(kx_segs s, const T *__restrict__ x,
+                                                         T *__restrict__ y, const double *sc,
+                                                         int nsc, int coef, double sign,
+                                                         int gate, double *partial)
+{
+    __shared__ double part[kBlock / kWave];
+    const int c = blockIdx.x;
+    const int p = s.chunk_seg[c];
+    double acc = 0.0;
+    if (seg_on(sc, nsc, p, gate)) {  // uniform over the block
+        double cf = sc[(size_t)p * nsc + coef];
+        cf = MODE == 2 ? 1.0 / cf : cf * sign;
+        const T ct = (T)cf;
+        for (int64_t i = s.chunk_lo[c] + threadIdx.x; i < s.chunk_lo[c + 1]; i += blockDim.x) {
+            T v;
+            if (MODE == 0)
+                v = x[i] + ct * y[i];
+            else if (MODE == 1)
+                v = y[i] + ct * x[i];
+            else
+                v = ct * y[i];
+            y[i] = v;
+            acc += (double)v * (double)v;
+        }
+    }
+    const double b = block_reduce(acc, part);
+    if (threadIdx.x == 0)
+        partial[c] = b;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void seg_dot_kernel(kx_segs s, const T *__restrict__ a,
+                                                         const T *__restrict__ b,
+                                                         const double *sc, int nsc, int gate,
+                                                         double *partial)
+{
+    __shared__ double part[kBlock / kWave];
+    const int c = blockIdx.x;
+    const int p = s.chunk_seg[c];
+    double acc = 0.0;
+    if (seg_on(sc, nsc, p, gate)) {
+        for (int64_t i = s.chunk_lo[c] + threadIdx.x; i < s.chunk_lo[c + 1]; i += blockDim.x)
+            acc += (double)a[i] * (double)b[i];
+    }
+    const double r = block_reduce(acc, part);
+    if (threadIdx.x == 0)
+        partial[c] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void seg_final_kernel(kx_segs s, const double *partial,
+                                                           double *sc, int nsc, int out,
+                                                           int gate)
+{
+    __shared__ double part[kBlock / kWave];
+    const int p = blockIdx.x;
+    if (!seg_on(sc, nsc, p, gate))
+        return;  // uniform over the block
+    double acc = 0.0;
+    for (int c = s.seg_chunk[p] + threadIdx.x; c < s.seg_chunk[p + 1]; c += blockDim.x)
+        acc += partial[c];
+    const double r = block_reduce(acc, part);
+    if (threadIdx.x == 0)
+        sc[(size_t)p * nsc + out] = r;
 }
 
 // LSQR scalar recurrences (CB:987-1136)
@@ -424,6 +514,114 @@ extern "C" int kx_lsqr_scalar(double *sc, double *cost, int32_t phase, void *str
         return kx_solver_fail(KX_EINVAL, "kx_lsqr_scalar: bad arguments");
     hipLaunchKernelGGL(lsqr_scalar_kernel, dim3(1), dim3(1), 0,
                        reinterpret_cast<hipStream_t>(stream), sc, cost, phase);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? KX_OK : kx_solver_fail((int)e, hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// segment-wise entry points
+// ---------------------------------------------------------------------------
+namespace {
+
+int check_segs(const kx_segs *s)
+{
+    if (!s || s->nseg <= 0 || s->nchunk <= 0 || !s->off || !s->chunk_seg || !s->chunk_lo ||
+        !s->seg_chunk)
+        return kx_solver_fail(KX_EINVAL, "segments: bad descriptor");
+    return KX_OK;
+}
+
+int seg_final(const kx_segs *s, double *partial, double *sc, int nsc, int out, int gate,
+              hipStream_t st)
+{
+    hipLaunchKernelGGL(seg_final_kernel, dim3(s->nseg), dim3(kBlock), 0, st, *s, partial, sc, nsc,
+                       out, gate);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? KX_OK : kx_solver_fail((int)e, hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" int64_t kx_seg_workspace_bytes(const kx_segs *s)
+{
+    return s && s->nchunk > 0 ? (int64_t)s->nchunk * 8 : -1;
+}
+
+extern "C" int kx_seg_vec(int32_t op, const kx_segs *s, int32_t dtype, const void *x, void *y,
+                          double *sc, int32_t nsc, int32_t coef, double sign, int32_t out,
+                          int32_t gate, void *work, void *stream)
+{
+    int rc = check_segs(s);
+    if (rc)
+        return rc;
+    if (!y || !sc || !work || coef < 0 || coef >= nsc || out >= nsc || gate >= nsc ||
+        (op != KX_VEC_SCALE && !x))
+        return kx_solver_fail(KX_EINVAL, "kx_seg_vec: bad arguments");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    double *partial = reinterpret_cast<double *>(work);
+    const dim3 grid(s->nchunk), block(kBlock);
+#define KX_SEG_VEC(T, M)                                                                       \
+    hipLaunchKernelGGL((seg_vec_kernel<T, M>), grid, block, 0, st, *s,                         \
+                       reinterpret_cast<const T *>(x), reinterpret_cast<T *>(y), sc, nsc, coef,  \
+                       sign, gate, partial)
+    if (dtype == KX_F32) {
+        if (op == KX_VEC_XPBY)
+            KX_SEG_VEC(float, 0);
+        else if (op == KX_VEC_AXPY)
+            KX_SEG_VEC(float, 1);
+        else
+            KX_SEG_VEC(float, 2);
+    } else if (dtype == KX_F64) {
+        if (op == KX_VEC_XPBY)
+            KX_SEG_VEC(double, 0);
+        else if (op == KX_VEC_AXPY)
+            KX_SEG_VEC(double, 1);
+        else
+            KX_SEG_VEC(double, 2);
+    } else {
+        return kx_solver_fail(KX_ETYPE, "kx_seg_vec: dtype");
+    }
+#undef KX_SEG_VEC
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return kx_solver_fail((int)e, hipGetErrorString(e));
+    return out >= 0 ? seg_final(s, partial, sc, nsc, out, gate, st) : KX_OK;
+}
+
+extern "C" int kx_seg_dot(const kx_segs *s, int32_t dtype, const void *a, const void *b,
+                          double *sc, int32_t nsc, int32_t out, int32_t gate, void *work,
+                          void *stream)
+{
+    int rc = check_segs(s);
+    if (rc)
+        return rc;
+    if (!a || !b || !sc || !work || out < 0 || out >= nsc || gate >= nsc)
+        return kx_solver_fail(KX_EINVAL, "kx_seg_dot: bad arguments");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    double *partial = reinterpret_cast<double *>(work);
+    if (dtype == KX_F32)
+        hipLaunchKernelGGL(seg_dot_kernel<float>, dim3(s->nchunk), dim3(kBlock), 0, st, *s,
+                           reinterpret_cast<const float *>(a), reinterpret_cast<const float *>(b),
+                           sc, nsc, gate, partial);
+    else if (dtype == KX_F64)
+        hipLaunchKernelGGL(seg_dot_kernel<double>, dim3(s->nchunk), dim3(kBlock), 0, st, *s,
+                           reinterpret_cast<const double *>(a),
+                           reinterpret_cast<const double *>(b), sc, nsc, gate, partial);
+    else
+        return kx_solver_fail(KX_ETYPE, "kx_seg_dot: dtype");
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return kx_solver_fail((int)e, hipGetErrorString(e));
+    return seg_final(s, partial, sc, nsc, out, gate, st);
+}
+
+extern "C" int kx_cgls_scalar_batch(double *sc, int32_t nseg, double *cost, double *cost1,
+                                    int32_t ncost, int32_t phase, void *stream)
+{
+    if (!sc || !cost || !cost1 || nseg <= 0 || ncost <= 0 || phase < 0 || phase > 2)
+        return kx_solver_fail(KX_EINVAL, "kx_cgls_scalar_batch: bad arguments");
+    hipLaunchKernelGGL(cgls_scalar_batch_kernel, dim3((nseg + kWave - 1) / kWave), dim3(kWave), 0,
+                       reinterpret_cast<hipStream_t>(stream), sc, nseg, cost, cost1, ncost, phase);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? KX_OK : kx_solver_fail((int)e, hipGetErrorString(e));
 }

@@ -1,4 +1,5 @@
 from . import basic  # noqa: F401
-from .basic import cgls, lsqr  # noqa: F401
+from .basic import cgls, cgls_batch, lsqr  # noqa: F401
 from .callback import Callbacks  # noqa: F401
 from .cls_basic import CGLS, LSQR  # noqa: F401
+from .cls_batch import CGLSBatch  # noqa: F401

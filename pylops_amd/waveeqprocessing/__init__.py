@@ -1,1 +1,2 @@
 from .kirchhoff import Kirchhoff  # noqa: F401
+from .kirchhoffbatch import KirchhoffBatch  # noqa: F401

@@ -37,14 +37,17 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= exported
 
 
-def test_plan_struct_layout_matches_header(tmp_path):
+@pytest.mark.parametrize("cname,pyname", [("kx_plan", "KxPlan"), ("kx_batch", "KxBatch"),
+                                           ("kx_segs", "KxSegs")])
+def test_struct_layout_matches_header(tmp_path, cname, pyname):
     """Compile the header with gcc and compare sizeof/offsetof with ctypes."""
     import ctypes
     from pylops_amd import _lib
-    fields = [f[0] for f in _lib.KxPlan._fields_]
+    cls = getattr(_lib, pyname)
+    fields = [f[0] for f in cls._fields_]
     prog = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
-            'int main(void){', 'printf("%zu\\n", sizeof(kx_plan));']
-    prog += [f'printf("%zu\\n", offsetof(kx_plan, {f}));' for f in fields]
+            'int main(void){', f'printf("%zu\\n", sizeof({cname}));']
+    prog += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
     prog += ['return 0;}']
     c = tmp_path / "layout.c"
     c.write_text("\n".join(prog))
@@ -52,9 +55,10 @@ def test_plan_struct_layout_matches_header(tmp_path):
     subprocess.run(["gcc", str(c), "-o", str(exe)], check=True)
     vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
                                            check=True).stdout.split()]
-    assert vals[0] == ctypes.sizeof(_lib.KxPlan)
+    assert vals[0] == ctypes.sizeof(cls)
+    assert len(vals) == len(fields) + 1
     for f, off in zip(fields, vals[1:]):
-        assert getattr(_lib.KxPlan, f).offset == off, f
+        assert getattr(cls, f).offset == off, f
 
 
 def test_header_constants_match_binding():
@@ -215,3 +219,23 @@ def test_linearoperator_semantics_numpy():
         Op.matvec(np.ones(5))
     np.testing.assert_allclose((2 * Op) @ np.ones(4), 2 * A @ np.ones(4))
     np.testing.assert_allclose((Op.H * Op) @ np.ones(4), A.T @ A @ np.ones(4))
+
+
+def test_segments_cover_each_segment_in_chunks():
+    """The chunk tables of the segment-wise solver kernels: every chunk lies
+    inside one segment, chunks tile each segment in order, empty segments
+    get one empty chunk."""
+    from pylops_amd.optimization.cls_batch import Segments
+    off = np.array([0, 5, 5, 40, 41, 100])
+    S = Segments(off, torch.device("cpu"), chunk=16)
+    lo = S._chunk_lo.numpy()
+    seg = S._chunk_seg.numpy()
+    first = S._seg_chunk.numpy()
+    assert S.nseg == 5 and S.nchunk == len(seg) == len(lo) - 1
+    assert first[0] == 0 and first[-1] == S.nchunk
+    assert lo[-1] == off[-1] and np.all(np.diff(lo) >= 0) and np.all(np.diff(lo) <= 16)
+    for p in range(S.nseg):
+        cs = range(first[p], first[p + 1])
+        assert len(cs) >= 1 and all(seg[c] == p for c in cs)
+        assert lo[first[p]] == off[p] and lo[first[p + 1]] == off[p + 1]
+    assert S.work.numel() == S.nchunk
