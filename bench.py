@@ -1,6 +1,6 @@
 """Kirchhoff forward + adjoint throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2d|3d|2dsmall]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2d|3d|2dsmall|batch]
                     [--tables f32|f16] [--no-cpu] [--extra] [--no-solve] [--no-batch]
 
 A step is one forward (spread + wavelet convolution) of a dense image and
@@ -19,6 +19,11 @@ Workloads (synthetic inputs, fixed seeds):
            20x20 receivers on the surface; kinematic.
   2dsmall  2D, 2000 m/s, kinematic; 100 x 101 at 10 m; nt=651 at 4 ms;
            11 sources, 51 receivers.
+  batch    64 independent 2D problems of uneven sizes (nz U[64,256], nx
+           U[64,512], nsrc U[4,32], nrec U[32,256], nt U[256,1024] at 4 ms,
+           velocity U[1500,3500] m/s, 15-30 Hz Ricker), fp16 tables; a step
+           is one batched forward + one batched adjoint of all problems;
+           also 20 CGLS iterations per problem, batched and one by one.
 
 Output: one JSON line.  value = 2 * ni * ns * nr / (t_fwd + t_adj) / 1e9
 (Gsamples/s, one sample = one image point x source x receiver triplet).
@@ -225,7 +230,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "2dsmall"])
+    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "2dsmall", "batch"])
     ap.add_argument("--tables", default="f32", choices=["f32", "f16"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true",
@@ -255,6 +260,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload == "batch":
+        print(json.dumps(batch_line(dev, args)), flush=True)
+        return
     rng = np.random.default_rng(1002)
     if args.workload == "2d":
         kw, refl, desc = workload_2d(rng)
@@ -346,7 +354,7 @@ def main():
     if args.extra:
         out["extra"] = extra_3d(dev)
     if not args.no_batch:
-        out["batch"] = time_batch(dev)
+        out["batch"] = time_batch(dev, sequential=not args.no_solve)[0]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(kw, desc, refl, dobs.cpu().numpy())
     print(json.dumps(out), flush=True)
@@ -403,65 +411,165 @@ def workload_batch(rng, nprob=64):
     return probs
 
 
-def time_batch(dev, niter=20):
-    """The batch of 64 uneven problems with fp16 tables: one batched
-    forward + adjoint (persistent launches), and 20 CGLS iterations per
-    problem, all problems at once (graph-replayed iterations) and, for
-    comparison, one problem after the other with the single-problem
-    solver."""
+def batch_model_bytes(ops):
+    """Separable-stream bytes of one application of every problem of a
+    batch with fp16 offset tables (2 B per entry plus 4 B bases per tile)."""
+    b = 0
+    for op in ops:
+        desc = {"ns": op.ns, "nr": op.nr, "nt": op.nt, "dynamic": op.dynamic}
+        b += algorithmic_bytes(desc, op.ni, 2)
+        ntiles = -(-op.ni // 64)
+        b += (op.ns + op.nr) * ntiles * 4
+    return b
+
+
+def time_batch(dev, steps=5, warmup=2, niter=20, sequential=True):
+    """The batch of 64 uneven problems with fp16 tables: batched forward +
+    adjoint (persistent launches, HIP events around each), and 20 CGLS
+    iterations per problem, all problems at once (graph-replayed
+    iterations) and, for comparison, one problem after the other with the
+    single-problem solver."""
     import torch
     import pylops_amd
     from pylops_amd.optimization.basic import cgls, cgls_batch
-    from pylops_amd.waveeqprocessing import KirchhoffBatch
+    from pylops_amd.waveeqprocessing import KirchhoffBatch, autotune_batch
     rng = np.random.default_rng(1004)
     probs = workload_batch(rng)
+    kws = [dict(kw, dtype="float32", tabledtype="float16") for kw, _ in probs]
+    # launch shapes from the autotuner (measured on this device, not cached)
     t0 = time.perf_counter()
-    ops = [pylops_amd.Kirchhoff(dtype="float32", tabledtype="float16", **kw) for kw, _ in probs]
-    B = KirchhoffBatch(ops)
+    tuned = autotune_batch(kws, reps=2, cache=False)
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ops = [pylops_amd.Kirchhoff(**kw, tile=tuned["tile"]) for kw in kws]
+    B = KirchhoffBatch(ops, rblock=tuned["rblock"], sorder=tuned["sorder"],
+                       gsplit=tuned["gsplit"])
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     m = torch.as_tensor(np.concatenate([r.ravel() for _, r in probs]), device=dev)
     y = torch.empty(B.shape[0], dtype=torch.float32, device=dev)
     a = torch.empty(B.shape[1], dtype=torch.float32, device=dev)
+    dout = torch.empty_like(y)
     B._into(m, y, False)
-    B._into(y, a, True)
-    torch.cuda.synchronize()
-    n = 5
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(n):
-        B._into(a, y, False)
+    B._into(y, a, True)   # dense model: the migrated image
+    for _ in range(warmup):
+        B._into(a, dout, False)
         B._into(y, a, True)
-    e1.record()
     torch.cuda.synchronize()
-    t_pair = e0.elapsed_time(e1) / n * 1e-3
-    B._into(m, y, False)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for e in evs:
+        e[0].record()
+        B._into(a, dout, False)
+        e[1].record()
+        B._into(y, a, True)
+        e[2].record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    t_fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) * 1e-3
+    t_adj = float(np.mean([e[1].elapsed_time(e[2]) for e in evs])) * 1e-3
     trip = float(sum(op.ni * op.ns * op.nr for op in ops))
+    res = {"workload": "lsm_batch_64_2d_uneven", "nprob": len(ops), "tables": "f16",
+           "rblock": B.rblock, "gsplit": B.gsplit, "sorder": B.sorder,
+           "spread_items": int(B._batch.n_spread), "gather_items": int(B._batch.n_gather),
+           "model_points": int(B.shape[1]), "data_samples": int(B.shape[0]),
+           "triplets": trip, "setup_s": round(t_setup, 3),
+           "autotune": {"tile": list(tuned["tile"]), "rblock": tuned["rblock"],
+                        "sorder": tuned["sorder"], "gsplit": tuned["gsplit"],
+                        "ms_fwd_adj": tuned["ms"], "tuning_s": round(t_tune, 2)},
+           "ms_per_step": round(wall * 1e3, 4),
+           "fwd_ms": round(t_fwd * 1e3, 4), "adj_ms": round(t_adj * 1e3, 4),
+           "fwd_adj_gsamples_s": round(2 * trip / wall / 1e9, 2),
+           "model_bytes": int(batch_model_bytes(ops))}
+    B._into(m, y, False)
     cgls_batch(B, y, niter=2, tol=0.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = cgls_batch(B, y, niter=niter, tol=0.0)
+    sol = cgls_batch(B, y, niter=niter, tol=0.0)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
-    ys = B.split_data(y)
-    t0 = time.perf_counter()
-    seq = [cgls(op, yp, niter=niter, tol=0.0, graph=True) for op, yp in zip(ops, ys)]
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
-    diff = max(float((rb[0].reshape(-1) - rs[0].reshape(-1)).norm() / rs[0].norm())
-               for rb, rs in zip(res, seq))
-    return {"workload": "lsm_batch_64_2d_uneven", "nprob": len(ops), "tables": "f16",
-            "niter": niter, "rblock": B.rblock, "gsplit": B.gsplit,
-            "spread_items": int(B._batch.n_spread), "gather_items": int(B._batch.n_gather),
-            "model_points": int(B.shape[1]), "data_samples": int(B.shape[0]),
-            "setup_s": round(t_setup, 3),
-            "fwd_adj_ms": round(t_pair * 1e3, 3),
-            "fwd_adj_gsamples_s": round(2 * trip / t_pair / 1e9, 2),
-            "cgls_s_per_batch_solve": round(t_batch, 4),
-            "cgls_s_sequential_solves": round(t_seq, 4),
-            "speedup_vs_sequential": round(t_seq / t_batch, 3),
-            "max_rel_diff_batch_vs_sequential_x": diff,
-            "iters": sorted(set(int(r[2]) for r in res))}
+    res.update({"niter": niter, "cgls_s_per_batch_solve": round(t_batch, 4),
+                "iters": sorted(set(int(r[2]) for r in sol))})
+    if sequential:
+        ys = B.split_data(y)
+        t0 = time.perf_counter()
+        for op, yp in zip(ops, ys):
+            cgls(op, yp, niter=niter, tol=0.0, graph=True)
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
+        # the single-problem adjoint picks its own split of a tile's sources
+        # over waves; with the batch's split it sums in the same order, and
+        # the solutions are bitwise those of the batch
+        same = True
+        for p in range(4):
+            ops[p].gsplit = B.gsplit
+            xs = cgls(ops[p], ys[p], niter=niter, tol=0.0, graph=True)[0]
+            same &= bool(torch.equal(xs.reshape(-1), sol[p][0].reshape(-1)))
+            ops[p].gsplit = 0
+        res.update({"cgls_s_sequential_solves": round(t_seq, 4),
+                    "speedup_vs_sequential": round(t_seq / t_batch, 3),
+                    "bitwise_equal_to_sequential_same_split": same})
+    return res, probs
+
+
+def cpu_baseline_batch(probs, budget_s=12.0):
+    """The oracle on a bounded sample of the batch: problems in order, one
+    forward + one adjoint each (float32 operator arithmetic, fp64 tables),
+    until the time budget is spent."""
+    from oracle import oracle as O
+    nthreads = O.lib().or_num_threads()
+    rng = np.random.default_rng(0)
+    trip, tsum, k = 0.0, 0.0, 0
+    for kw, refl in probs:
+        kwo = dict(kw)
+        kwo["dtype"] = "float32"
+        orc = O.KirchhoffOracle(**kwo)
+        d = rng.standard_normal(orc.ns * orc.nr * orc.nt).astype(np.float32)
+        t0 = time.perf_counter()
+        orc.matvec(refl.ravel())
+        orc.rmatvec(d)
+        tsum += time.perf_counter() - t0
+        trip += 2.0 * orc.ni * orc.ns * orc.nr
+        k += 1
+        if tsum > budget_s:
+            break
+    return dict(value=trip / tsum / 1e9, unit="Gsamples/s", cores=int(nthreads), kind="port",
+                sample=f"lsm_batch_64_2d_uneven, the first {k} of {len(probs)} problems, "
+                       f"one forward + one adjoint each (float32 operator arithmetic), "
+                       f"{tsum:.1f} s")
+
+
+def batch_line(dev, args):
+    res, probs = time_batch(dev, steps=args.steps, warmup=args.warmup,
+                            sequential=not args.no_solve)
+    t_dom = max(res["fwd_ms"], res["adj_ms"]) * 1e-3
+    dom = "batch_forward" if res["fwd_ms"] >= res["adj_ms"] else "batch_adjoint"
+    achieved = res["model_bytes"] / t_dom / 1e9
+    out = {
+        "metric": METRIC,
+        "value": res["fwd_adj_gsamples_s"],
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": res["workload"], "tables": "f16",
+                   "step": "batched forward of K^T d + batched adjoint of d, 64 problems",
+                   "batch": res},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(res["workload"], dom),
+                     "model_bytes": res["model_bytes"]},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(probs)
+    return out
 
 
 def extra_3d(dev):

@@ -1,2 +1,3 @@
 from .kirchhoff import Kirchhoff  # noqa: F401
 from .kirchhoffbatch import KirchhoffBatch  # noqa: F401
+from .autotune import autotune, autotune_batch  # noqa: F401

@@ -30,8 +30,9 @@ logger = logging.getLogger(__name__)
 
 _ENGINES = ("hip", "cuda", "numba", "numpy")
 # small attributes whose reassignment invalidates the device tables
-_PLAIN_ATTRS = ("vel", "aperture", "angleaperture", "six", "rix", "tile", "rblock",
-                "tabledtype")
+_PLAIN_ATTRS = ("vel", "aperture", "angleaperture", "six", "rix", "tile", "tabledtype")
+# launch-shape attributes: copied into the plan, the tables stay
+_LAUNCH_ATTRS = ("rblock", "sorder", "gsplit")
 
 
 class _Table:
@@ -75,6 +76,12 @@ class Kirchhoff(LinearOperator):
     tile : ``(tz, tx, ty)`` image tile of 64 points.
     device : torch device of the tables (default: current HIP device).
     rblock : receivers per forward workgroup (0 = automatic).
+    sorder : forward lane order, ``KX_ORDER_NATURAL`` (1) or
+        ``KX_ORDER_ROTATED`` (2); 0 = automatic.
+    gsplit : adjoint waves per tile, 1, 2 or 4; 0 = automatic.
+
+    ``waveeqprocessing.autotune`` measures the fastest ``tile``, ``rblock``,
+    ``sorder`` and ``gsplit`` for a geometry.
     """
 
     trav_srcs = _Table()
@@ -89,7 +96,7 @@ class Kirchhoff(LinearOperator):
                  mode="eikonal", wavfilter=False, dynamic=False, trav=None, amp=None,
                  aperture=None, angleaperture=90.0, snell=None, engine="hip",
                  dtype="float64", name="K", *, tabledtype=None, tile=None,
-                 device=None, rblock=0):
+                 device=None, rblock=0, sorder=0, gsplit=0):
         object.__setattr__(self, "_dirty", True)
         object.__setattr__(self, "_tab", {})
         object.__setattr__(self, "_tab_host", {})
@@ -201,6 +208,8 @@ class Kirchhoff(LinearOperator):
                 raise NotImplementedError(f"tabledtype {self.tabledtype} with dtype {self.dtype}")
         self.tile = tuple(tile) if tile is not None else T.default_tile(self.ndims, self.nz)
         self.rblock = int(rblock)
+        self.sorder = int(sorder)
+        self.gsplit = int(gsplit)
         self.engine = engine
         if engine == "cuda" and not self.travsrcrec:
             raise NotImplementedError("engine='cuda' not implemented for traveltimes "
@@ -219,6 +228,8 @@ class Kirchhoff(LinearOperator):
         object.__setattr__(self, name, value)
         if name in _PLAIN_ATTRS:
             object.__setattr__(self, "_dirty", True)
+        elif name in _LAUNCH_ATTRS and "_plan" in self.__dict__:
+            setattr(self._plan, name, int(value))
 
     @property
     def itrav(self):
@@ -352,6 +363,8 @@ class Kirchhoff(LinearOperator):
         plan.ns, plan.nr, plan.nt = ns, nr, self.nt
         plan.neg_trav = int(neg)
         plan.rblock = self.rblock
+        plan.sorder = self.sorder
+        plan.gsplit = self.gsplit
         plan.nh = keep["h"].numel()
         for k, v in keep.items():
             setattr(plan, k, v.data_ptr())

@@ -217,3 +217,52 @@ def test_batch_rejects_mixed_table_formats():
     b = pylops_amd.Kirchhoff(**_problem(rng, False, "float32", "float16"))
     with pytest.raises(ValueError, match="differs from problem 0"):
         KirchhoffBatch([a, b])
+
+
+def test_autotune_single_operator(tmp_path):
+    """The tuner returns a valid shape set, caches it, and the tuned
+    operator gives the default operator's results."""
+    import pylops_amd
+    from pylops_amd.waveeqprocessing import autotune
+    rng = np.random.default_rng(31)
+    kw = _problem(rng, True, "float32")
+    cache = str(tmp_path / "tune.json")
+    best = autotune(cache=cache, reps=1, **kw)
+    assert best["tile"][0] * best["tile"][1] * best["tile"][2] == 64
+    assert best["rblock"] in (1, 2, 4, 8) and best["gsplit"] in (1, 2, 4)
+    assert best["ms"] > 0
+    assert autotune(cache=cache, reps=1, **kw) == best  # from the cache
+    op0 = pylops_amd.Kirchhoff(**kw)
+    op = pylops_amd.Kirchhoff(**kw, tile=best["tile"], rblock=best["rblock"],
+                              sorder=best["sorder"], gsplit=best["gsplit"])
+    m = rng.standard_normal(op.shape[1]).astype(np.float32)
+    d = rng.standard_normal(op.shape[0]).astype(np.float32)
+    np.testing.assert_array_equal(op @ m, op0 @ m)
+    assert G.relerr(op.H @ d, op0.H @ d) < 1e-6
+
+
+def test_autotune_batch(tmp_path):
+    import pylops_amd
+    from pylops_amd.waveeqprocessing import KirchhoffBatch, autotune_batch
+    rng = np.random.default_rng(32)
+    kws = [_problem(rng, False, "float32", "float16") for _ in range(4)]
+    best = autotune_batch(kws, reps=1, cache=False)
+    ops = [pylops_amd.Kirchhoff(**kw, tile=best["tile"]) for kw in kws]
+    B = KirchhoffBatch(ops, rblock=best["rblock"], sorder=best["sorder"], gsplit=best["gsplit"])
+    assert B.rblock == best["rblock"] and B.gsplit == best["gsplit"]
+    B0 = KirchhoffBatch([pylops_amd.Kirchhoff(**kw) for kw in kws])
+    m = rng.standard_normal(B.shape[1]).astype(np.float32)
+    d = rng.standard_normal(B.shape[0]).astype(np.float32)
+    np.testing.assert_array_equal(B @ m, B0 @ m)
+    assert G.relerr(B.H @ d, B0.H @ d) < 1e-6
+
+
+def test_launch_attributes_update_the_plan():
+    import pylops_amd
+    rng = np.random.default_rng(33)
+    op = pylops_amd.Kirchhoff(**_problem(rng, False, "float32"))
+    m = rng.standard_normal(op.shape[1]).astype(np.float32)
+    d0 = op @ m
+    op.rblock, op.sorder, op.gsplit = 2, 1, 4
+    assert (op.plan.rblock, op.plan.sorder, op.plan.gsplit) == (2, 1, 4)
+    np.testing.assert_array_equal(op @ m, d0)
