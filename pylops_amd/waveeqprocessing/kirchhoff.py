@@ -424,6 +424,38 @@ class Kirchhoff(LinearOperator):
     def _rmatvec(self, x):
         return self._launch(_lib.load().kx_adjoint, x, self.ni, "kx_adjoint")
 
+    # several columns at once: the reference's matmat applies matvec column
+    # by column (LO:436-447); here the k columns run as a batch of k copies
+    # of this operator, one persistent launch per kernel sharing the tables
+    def _copies(self, k):
+        cache = self.__dict__.get("_mm")
+        if cache is None or cache[0] != k or cache[1]._keep[0] is not self._dev_tables:
+            from pylops_amd.waveeqprocessing.kirchhoffbatch import KirchhoffBatch
+            _ = self.plan  # stale tables are rebuilt first
+            B = KirchhoffBatch([self] * k, rblock=self.rblock, sorder=self.sorder,
+                               gsplit=self.gsplit)
+            object.__setattr__(self, "_mm", (k, B))
+        return self._mm[1]
+
+    def _multi(self, X, adjoint):
+        k = X.shape[1]
+        nout = self.shape[1] if adjoint else self.shape[0]
+        B = self._copies(k)
+        xd = to_device(X, self.dtype, self.device).T.contiguous().reshape(-1)
+        out = torch.empty(k * nout, dtype=xd.dtype, device=self.device)
+        B._into(xd, out, adjoint)
+        return like_input(out.reshape(k, nout).T, X)
+
+    def _matmat(self, X):
+        if X.shape[1] == 1:
+            return super()._matmat(X)
+        return self._multi(X, False)
+
+    def _rmatmat(self, X):
+        if X.shape[1] == 1:
+            return super()._rmatmat(X)
+        return self._multi(X, True)
+
     # kernels alone, without the wavelet (the engine seam, K:1003-1046)
     def spread(self, m):
         """The forward kernel alone (no wavelet): traces (ns*nr, nt)."""

@@ -87,3 +87,50 @@ def test_gradient_finite_differences(dtype):
         errs.append(abs(fd - an) / max(abs(an), 1e-12 * float(g.norm())))
     tol = 1e-6 if dtype == "float64" else 2e-2
     assert np.median(errs) < tol, errs
+
+
+def test_matmat_runs_columns_as_one_batch():
+    """Kirchhoff.matmat / rmatmat run k columns as one batched launch: the
+    float32 forward columns are bitwise the matvecs, the adjoint columns
+    equal the rmatvecs to summation-order rounding."""
+    Op = _kop("float32")
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((Op.shape[1], 5)).astype(np.float32)
+    V = rng.standard_normal((Op.shape[0], 5)).astype(np.float32)
+    Y = Op.matmat(X)
+    Z = Op.rmatmat(V)
+    assert Y.shape == (Op.shape[0], 5) and Z.shape == (Op.shape[1], 5)
+    for j in range(5):
+        np.testing.assert_array_equal(Y[:, j], Op.matvec(X[:, j]))
+        assert G.relerr(Z[:, j], Op.rmatvec(V[:, j])) < 1e-6
+    # the cached batch follows table reassignment
+    Op.trav_recs = Op.trav_recs * 1.01
+    Y2 = Op.matmat(X)
+    np.testing.assert_array_equal(Y2[:, 1], Op.matvec(X[:, 1]))
+    assert G.relerr(Y2, Y) > 1e-3
+
+
+def test_torchoperator_batch_kirchhoff_vs_oracle():
+    """pytests/test_torchoperator.py:61-80 on the Kirchhoff operator: the
+    batched forward equals matmat exactly; the gradient equals rmatmat
+    exactly and the oracle's adjoint (the reference's TorchOperator
+    gradient is its adjoint) to 1e-5."""
+    import torch
+    from oracle import oracle as O
+    from pylops_amd import TorchOperator
+    Op = _kop("float32")
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dynamic=True, dtype="float32")
+    orc = O.KirchhoffOracle(**kw)
+    Top = TorchOperator(Op, batch=True, device="gpu")
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((4, Op.shape[1])).astype(np.float32)
+    v = rng.standard_normal((4, Op.shape[0])).astype(np.float32)
+    xt = torch.as_tensor(x, device="cuda").requires_grad_(True)
+    yt = Top.apply(xt)
+    np.testing.assert_array_equal(yt.detach().cpu().numpy(), Op.matmat(x.T).T)
+    yt.backward(torch.as_tensor(v, device="cuda"))
+    g = xt.grad.cpu().numpy()
+    np.testing.assert_array_equal(g, Op.rmatmat(v.T).T)
+    for j in range(4):
+        assert G.relerr(g[j], orc.rmatvec(v[j].astype(np.float64))) < 1e-5
