@@ -26,8 +26,8 @@ Workloads (synthetic inputs, fixed seeds):
            also 20 CGLS iterations per problem, batched and one by one.
   autograd the 2d operator as a torch autograd Function on 16 models: loss
            0.5|Km-d|^2 + 1e-3 TV(m), 5% noise, 50 Adam steps (a step is
-           one batched forward + backward + Adam), and a central-difference
-           check of the gradient along 32 directions.
+           one batched forward + backward + Adam, --adam-steps), and a
+           central-difference check of the gradient along 32 directions.
 
 Output: one JSON line.  value = 2 * ni * ns * nr / (t_fwd + t_adj) / 1e9
 (Gsamples/s, one sample = one image point x source x receiver triplet).
@@ -241,6 +241,8 @@ def main():
                     help="also time the 3D workload with float32 and float16 tables")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip timing of the 30-iteration CGLS / LSQR solves")
+    ap.add_argument("--adam-steps", type=int, default=50,
+                    help="Adam steps of the autograd workload")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batch of 64 uneven 2D problems (batched CGLS)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -268,7 +270,7 @@ def main():
         print(json.dumps(batch_line(dev, args)), flush=True)
         return
     if args.workload == "autograd":
-        print(json.dumps(autograd_line(dev, args)), flush=True)
+        print(json.dumps(autograd_line(dev, args, nsteps=args.adam_steps)), flush=True)
         return
     rng = np.random.default_rng(1002)
     if args.workload == "2d":
@@ -589,12 +591,13 @@ def isotropic_tv(m, eps=1e-8):
 
 def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
     """The 2D dynamic operator (601x300, 61x301, nt=1501) as a torch
-    autograd Function on a batch of 16 models: loss = 0.5 |K m - d|^2 +
+    autograd Function on a batch of 16 models: loss = 0.5 |(K m - d)/s|^2 +
     1e-3 TV(m) against data from 16 seeded ground-truth models with 5%
-    Gaussian noise, 50 Adam steps from zero; the gradient is checked
-    against central differences (eps=1e-3) along 32 random unit
-    directions.  One step = batched forward + batched adjoint of all 16
-    models (one persistent launch each) + TV and Adam."""
+    Gaussian noise (s = the clean data's standard deviation), 50 Adam steps
+    from zero.  The gradient of the data term (the adjoint kernel) is
+    checked against central differences along 32 random unit directions
+    with step 1e-3 |m|.  One step = batched forward + batched adjoint of all
+    16 models (one persistent launch each) + TV and Adam."""
     import torch
     import pylops_amd
     from pylops_amd import TorchOperator
@@ -608,19 +611,22 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
                             dtype=torch.float32)
     with torch.no_grad():
         dclean = Top.apply(mtrue)
+        sig = float(dclean.std())
         g = torch.Generator(device="cpu").manual_seed(5)
         noise = torch.randn(dclean.shape, generator=g).to(dev)
-        dobs = dclean + 0.05 * dclean.std() * noise
+        dobs = (dclean + 0.05 * sig * noise) / sig
     lam = 1e-3
 
+    def data_term(m):
+        r = Top.apply(m) / sig - dobs
+        return 0.5 * (r.double() ** 2).sum()
+
     def loss_fn(m):
-        r = Top.apply(m) - dobs
-        return 0.5 * (r.double() ** 2).sum() + lam * isotropic_tv(m.view(-1, nx, nz)).double()
+        return data_term(m) + lam * isotropic_tv(m.view(-1, nx, nz)).double()
 
     m = torch.zeros_like(mtrue, requires_grad=True)
     opt = torch.optim.Adam([m], lr=0.02)
-    # warm-up: the batched operator and torch's kernels
-    for _ in range(args.warmup):
+    for _ in range(args.warmup):  # the batched operator and torch's kernels
         opt.zero_grad()
         loss_fn(m).backward()
     m.data.zero_()
@@ -632,8 +638,9 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
     for k in range(nsteps):
         opt.zero_grad()
         ev[k][0].record()
-        r = Top.apply(m) - dobs
+        pred = Top.apply(m)
         ev[k][1].record()
+        r = pred / sig - dobs
         loss = 0.5 * (r.double() ** 2).sum() + lam * isotropic_tv(m.view(-1, nx, nz)).double()
         loss.backward()
         ev[k][2].record()
@@ -644,21 +651,24 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
     t_fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in ev])) * 1e-3
     t_bwd = float(np.mean([e[1].elapsed_time(e[2]) for e in ev])) * 1e-3
     losses = [float(v) for v in losses]
-    # gradient check at the final model
+    err_true = float((m.detach() - mtrue).norm() / mtrue.norm())
+    # adjoint-kernel gradient vs central differences of the data term; the
+    # difference of the two losses is summed as (r+ - r-)(r+ + r-) / 2 so that
+    # it does not cancel
     m0 = m.detach().clone().requires_grad_(True)
-    loss_fn(m0).backward()
+    data_term(m0).backward()
     gr = m0.grad.double()
-    eps = 1e-3
+    h = 1e-3 * float(m0.detach().norm())
     errs = []
-    for _ in range(ndir):
-        u = torch.as_tensor(rng.standard_normal(m0.shape), dtype=torch.float64, device=dev)
-        u /= u.norm()
-        with torch.no_grad():
-            lp = loss_fn((m0.double() + eps * u).float())
-            lm = loss_fn((m0.double() - eps * u).float())
-        fd = float((lp - lm) / (2 * eps))
-        an = float((gr * u).sum())
-        errs.append(abs(fd - an) / max(abs(an), 1e-30))
+    with torch.no_grad():
+        for _ in range(ndir):
+            u = torch.as_tensor(rng.standard_normal(m0.shape), dtype=torch.float64, device=dev)
+            u /= u.norm()
+            rp = (Top.apply((m0.double() + h * u).float()) / sig - dobs).double()
+            rm = (Top.apply((m0.double() - h * u).float()) / sig - dobs).double()
+            fd = float((0.5 * (rp - rm) * (rp + rm)).sum() / (2 * h))
+            an = float((gr * u).sum())
+            errs.append(abs(fd - an) / max(abs(an), 1e-30))
     trip = float(Op.ni) * Op.ns * Op.nr * nbatch
     step_s = wall / nsteps
     bytes_alg = algorithmic_bytes(desc, Op.ni, 4) * nbatch
@@ -676,13 +686,17 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
         "dtype": "f32",
         "data": "synthetic",
         "config": {**desc, "workload": "autograd_2d_dynamic_batch16_tv_adam",
-                   "batch": nbatch, "tv_weight": lam, "noise": 0.05, "optimizer": "Adam lr=0.02",
+                   "batch": nbatch, "tv_weight": lam, "noise": 0.05,
+                   "optimizer": "Adam lr=0.02",
                    "step": "forward of 16 models + backward (adjoint of 16 residuals, TV) + Adam",
                    "forward_ms": round(t_fwd * 1e3, 3),
                    "loss_backward_ms": round(t_bwd * 1e3, 3),
-                   "s_per_50_steps": round(wall, 3),
+                   "s_total": round(wall, 3),
+                   "launch_shapes": {"rblock": Op._mm[1].rblock, "sorder": Op._mm[1].sorder,
+                                     "gsplit": Op._mm[1].gsplit},
                    "loss_first": losses[0], "loss_last": losses[-1],
-                   "fd_check": {"eps": eps, "directions": ndir,
+                   "model_rel_err_last": err_true,
+                   "fd_check": {"step": "1e-3 |m|", "directions": ndir, "term": "data",
                                 "median_rel_err": float(np.median(errs)),
                                 "max_rel_err": float(np.max(errs))}},
         "roofline": {"bound": "hbm", "kernel": "batched forward (16 models)",

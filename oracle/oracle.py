@@ -15,6 +15,8 @@ Contents (reference file:line each part follows):
   same loop nests as K 683-1001), orchestration K 1048-1143
 * ``conv_same`` -- Convolve1D short-filter path, convolve1d.py 57-115
 * ``cgls`` / ``lsqr`` -- pylops/optimization/cls_basic.py 418-708, 835-1351
+* ``ista`` / ``fista`` and their thresholds -- pylops/optimization/
+  cls_sparsity.py 36-200, 1313-2160 (synthesis form, no SOp)
 * ``dottest`` -- pylops/utils/dottest.py 10-132
 * ``ricker`` / ``aperture_taper`` -- pylops/utils/wavelets.py 167-206,
   pylops/utils/tapers.py 18-56, 115-151
@@ -590,6 +592,67 @@ def lsqr(Op, y, x0=None, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8,
         iiter += 1
     return (x, istop, iiter, r1norm, r2norm, anorm, acond, arnorm, xnorm, var,
             np.array(cost))
+
+
+def soft_threshold(x, thresh):
+    """cls_sparsity.py:64-93 (real case)."""
+    return np.maximum(np.abs(x) - thresh, 0.0) * np.sign(x)
+
+
+def hard_threshold(x, thresh):
+    """cls_sparsity.py:36-61."""
+    x1 = x.copy()
+    x1[np.abs(x) <= np.sqrt(2 * thresh)] = 0
+    return x1
+
+
+def half_threshold(x, thresh):
+    """cls_sparsity.py:96-133 (the closed-form proximal map of
+    thresh * |u|^(1/2), Chen, Shen and Suter 2016)."""
+    arg = np.ones_like(x)
+    nz = x != 0
+    arg[nz] = (thresh / 8.0) * (np.abs(x[nz]) / 3.0) ** (-1.5)
+    arg = np.clip(arg, -1, 1)
+    phi = 2.0 / 3.0 * np.arccos(arg)
+    x1 = 2.0 / 3.0 * x * (1 + np.cos(2.0 * np.pi / 3.0 - phi))
+    x1[np.abs(x) <= (54 ** (1.0 / 3.0) / 4.0) * thresh ** (2.0 / 3.0)] = 0
+    return x1
+
+
+_THRESH = {"soft": soft_threshold, "hard": hard_threshold, "half": half_threshold}
+
+
+def shrinkage(Op, y, alpha, niter=10, eps=0.1, tol=1e-10, threshkind="soft", x0=None,
+              fast=True, decay=None):
+    """ISTA (fast=False, cls_sparsity.py:1700-1800) or FISTA (fast=True,
+    cls_sparsity.py:2010-2120) with a given step ``alpha``; returns
+    ``(x, niter, cost)``."""
+    thr = _THRESH[threshkind]
+    thresh = eps * alpha * 0.5
+    x = np.zeros(Op.shape[1]) if x0 is None else np.asarray(x0, dtype=np.float64).copy()
+    z = x.copy()
+    t = 1.0
+    cost = []
+    it = 0
+    xupdate = np.inf
+    while it < niter and xupdate > tol:
+        xold = x.copy()
+        base = z if fast else x
+        res = y - Op.matvec(base)
+        xu = base + alpha * Op.rmatvec(res)
+        d = 1.0 if decay is None else decay[it]
+        x = thr(xu, d * thresh)
+        if fast:
+            told = t
+            t = (1.0 + np.sqrt(1.0 + 4.0 * t ** 2)) / 2.0
+            z = x + ((told - 1.0) / t) * (x - xold)
+            costdata = 0.5 * np.linalg.norm(y - Op.matvec(x)) ** 2
+        else:
+            costdata = 0.5 * np.linalg.norm(res) ** 2
+        xupdate = np.linalg.norm(x - xold)
+        cost.append(costdata + eps * np.linalg.norm(x, ord=1))
+        it += 1
+    return x, it, np.array(cost)
 
 
 def dottest_values(Op, seed=0, dtype=np.float64):

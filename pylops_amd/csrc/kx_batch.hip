@@ -5,18 +5,26 @@
 // problems of uneven sizes are inverted together.  Every operator application
 // of a batch is one persistent launch per engine kernel: a grid sized to fill
 // the chip pulls work items -- (problem, source, receiver block) for the
-// spread, (problem, tile group) for the gather -- from one atomic counter, in
-// order of decreasing cost, so that small and large problems share the CUs
-// without a launch per problem.  The item bodies are the single-problem ones
-// (kx_device.h), so a problem's results do not depend on the batch it is in.
+// spread, (problem, tile group) for the gather -- from atomic counters, so
+// that small and large problems share the CUs without a launch per problem.
+// The item bodies are the single-problem ones (kx_device.h), so a problem's
+// results do not depend on the batch it is in.
+//
+// Locality: each XCD has its own L2.  Items that read the same table rows --
+// one receiver block of a problem (spread), a run of neighbouring tiles
+// (gather), and the same items of problems sharing their tables (the columns
+// of a multi-column product) -- form a group; groups are dealt to eight
+// per-XCD queues (largest first, to the least loaded queue), and workgroup b,
+// which the hardware places on XCD b mod 8, drains queue b mod 8 before
+// taking items from the others.
 //
 // Device memory (caller-owned, kx_batch_dev_bytes), 256-byte aligned parts:
 //   BatchProb   probs[nprob]       geometry, tables, offsets into m and d
 //   SpreadScale scales[nprob]      fixed-point scale of each problem's forward
 //   float       part[nprob][kBatchScaleBlocks]  partial max|m|
-//   unsigned    ctr[kCounters]     work counters (spread, gather), left zero
-//   SpreadItem  sitems[n_spread]   decreasing cost
-//   GatherItem  gitems[n_gather]   decreasing cost
+//   unsigned    ctr[kCounters]     per-XCD work counters and queue offsets
+//   SpreadItem  sitems[n_spread]   the eight spread queues, one after the other
+//   GatherItem  gitems[n_gather]   the eight gather queues
 //   int64_t     cprefix[nprob + 1] convolution blocks before each problem
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -25,7 +33,9 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <numeric>
+#include <utility>
 #include <vector>
 
 #include "kirchx.h"
@@ -62,7 +72,11 @@ int bfail(int code, const char *fmt, ...)
     } while (0)
 
 constexpr int kBatchScaleBlocks = 32;
-constexpr int kCounters = 16;  // [0, 1] spread next / done, [2, 3] gather
+constexpr int kXcds = 8;
+// counters: [0, 8) spread next per queue, [8] spread done, [16, 24) gather
+// next, [24] gather done; queue offsets: [32, 41) spread, [48, 57) gather
+constexpr int kCounters = 64;
+constexpr int kSpreadQ = 0, kGatherQ = 16, kQoff = 32;
 constexpr int kGatherFill = 24;
 constexpr int kLdsPerCu = 160 * 1024;
 constexpr int kMaxWavesPerCu = 32;
@@ -89,12 +103,27 @@ struct GatherItem {
 // ---------------------------------------------------------------------------
 // persistent work loop
 // ---------------------------------------------------------------------------
-// Next item of the launch: thread 0 takes it from the counter, the block
-// reads it from LDS.  Returns the same value in every lane.
-__device__ __forceinline__ int next_item(unsigned *ctr, int *cur)
+// Next item of the launch, or -1 when every queue is empty.  Thread 0 takes
+// it from the current queue's counter, moving on to the next queue when that
+// one is exhausted (q and left live in thread 0 only); the block reads the
+// item from LDS.  ctr: the launch's counters (next[8], done), qoff its
+// queue offsets.
+__device__ __forceinline__ int next_item(unsigned *ctr, const int *qoff, int *cur, int &q,
+                                         int &left)
 {
-    if (threadIdx.x == 0)
-        *cur = (int)atomicAdd(ctr, 1u);
+    if (threadIdx.x == 0) {
+        int it = -1;
+        while (left > 0) {
+            const unsigned k = atomicAdd(&ctr[q], 1u);
+            if (k < (unsigned)(qoff[q + 1] - qoff[q])) {
+                it = qoff[q] + (int)k;
+                break;
+            }
+            q = (q + 1) & (kXcds - 1);
+            --left;
+        }
+        *cur = it;
+    }
     __syncthreads();
     const int it = __builtin_amdgcn_readfirstlane(*cur);
     __syncthreads();  // read by every wave before thread 0 takes the next
@@ -102,14 +131,14 @@ __device__ __forceinline__ int next_item(unsigned *ctr, int *cur)
 }
 
 // The last workgroup to finish resets the counters for the next launch: by
-// then every workgroup has taken its final item.
+// then every workgroup has found every queue empty.
 __device__ __forceinline__ void launch_done(unsigned *ctr)
 {
     if (threadIdx.x == 0) {
         __threadfence();
-        if (atomicAdd(&ctr[1], 1u) == gridDim.x - 1) {
-            atomicExch(&ctr[0], 0u);
-            atomicExch(&ctr[1], 0u);
+        if (atomicAdd(&ctr[kXcds], 1u) == gridDim.x - 1) {
+            for (int k = 0; k <= kXcds; ++k)
+                atomicExch(&ctr[k], 0u);
         }
     }
 }
@@ -117,13 +146,14 @@ __device__ __forceinline__ void launch_done(unsigned *ctr)
 template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
 __global__ __launch_bounds__(kSpreadBlock) void batch_spread_kernel(
     const BatchProb *__restrict__ probs, const SpreadScale *__restrict__ scales,
-    const SpreadItem *__restrict__ items, int nitems, unsigned *ctr, const TD *__restrict__ m,
-    TD *__restrict__ y)
+    const SpreadItem *__restrict__ items, unsigned *ctr, const int *__restrict__ qoff,
+    const TD *__restrict__ m, TD *__restrict__ y)
 {
     __shared__ int cur;
+    int q = blockIdx.x & (kXcds - 1), left = kXcds;
     for (;;) {
-        const int it = next_item(ctr, &cur);
-        if (it >= nitems)
+        const int it = next_item(ctr, qoff, &cur, q, left);
+        if (it < 0)
             break;
         const SpreadItem w = items[it];
         const KArgs a = probs[w.p].a;
@@ -136,13 +166,14 @@ __global__ __launch_bounds__(kSpreadBlock) void batch_spread_kernel(
 
 template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
 __global__ __launch_bounds__(kGatherBlock) void batch_gather_kernel(
-    const BatchProb *__restrict__ probs, const GatherItem *__restrict__ items, int nitems,
-    unsigned *ctr, const TD *__restrict__ d, TD *__restrict__ m)
+    const BatchProb *__restrict__ probs, const GatherItem *__restrict__ items, unsigned *ctr,
+    const int *__restrict__ qoff, const TD *__restrict__ d, TD *__restrict__ m)
 {
     __shared__ int cur;
+    int q = blockIdx.x & (kXcds - 1), left = kXcds;
     for (;;) {
-        const int it = next_item(ctr, &cur);
-        if (it >= nitems)
+        const int it = next_item(ctr, qoff, &cur, q, left);
+        if (it < 0)
             break;
         const GatherItem w = items[it];
         const KArgs a = probs[w.p].a;
@@ -323,6 +354,48 @@ Fns fns_of(int dtype, int tab_fmt)
     return {spread_of<float, float>, gather_of<float, float>};
 }
 
+// Deals groups of work items to the eight XCD queues: largest group first,
+// each to the least loaded queue; a queue lists its groups in that order and
+// each group's items in `within` order.  Returns the item permutation and the
+// queue offsets qoff[0..8].
+template <typename Within>
+std::vector<int> deal(const std::vector<int> &group_of, int ngroups,
+                      const std::vector<int64_t> &cost, int qoff[kXcds + 1], Within within)
+{
+    std::vector<int64_t> gsum(ngroups, 0);
+    std::vector<std::vector<int>> members(ngroups);
+    for (size_t i = 0; i < group_of.size(); ++i) {
+        gsum[group_of[i]] += cost[i];
+        members[group_of[i]].push_back((int)i);
+    }
+    std::vector<int> gorder(ngroups);
+    std::iota(gorder.begin(), gorder.end(), 0);
+    std::stable_sort(gorder.begin(), gorder.end(),
+                     [&](int a, int b) { return gsum[a] > gsum[b]; });
+    int64_t load[kXcds] = {0};
+    std::vector<int> queue_groups[kXcds];
+    for (int g : gorder) {
+        int k = 0;
+        for (int j = 1; j < kXcds; ++j)
+            if (load[j] < load[k])
+                k = j;
+        load[k] += gsum[g];
+        queue_groups[k].push_back(g);
+    }
+    std::vector<int> perm;
+    perm.reserve(group_of.size());
+    for (int k = 0; k < kXcds; ++k) {
+        qoff[k] = (int)perm.size();
+        for (int g : queue_groups[k]) {
+            std::vector<int> &m = members[g];
+            std::sort(m.begin(), m.end(), within);
+            perm.insert(perm.end(), m.begin(), m.end());
+        }
+    }
+    qoff[kXcds] = (int)perm.size();
+    return perm;
+}
+
 int check_batch(const kx_plan *plans, int nprob)
 {
     if (!plans || nprob <= 0)
@@ -455,6 +528,7 @@ struct View {
     SpreadScale *scales;
     float *part;
     unsigned *ctr;
+    const int *ctr_const;  // the same words, read as queue offsets
     const SpreadItem *sitems;
     const GatherItem *gitems;
     const int64_t *cprefix;
@@ -469,6 +543,7 @@ View view(const kx_batch *b)
     v.scales = reinterpret_cast<SpreadScale *>(base + L.scales);
     v.part = reinterpret_cast<float *>(base + L.part);
     v.ctr = reinterpret_cast<unsigned *>(base + L.ctr);
+    v.ctr_const = reinterpret_cast<const int *>(base + L.ctr);
     v.sitems = reinterpret_cast<const SpreadItem *>(base + L.sitems);
     v.gitems = reinterpret_cast<const GatherItem *>(base + L.gitems);
     v.cprefix = reinterpret_cast<const int64_t *>(base + L.cprefix);
@@ -491,8 +566,9 @@ int launch_bspread(const kx_batch *b, const View &v, const void *m, void *y, hip
     if constexpr (RB <= max_rblock<TD, DYN>()) {
         hipLaunchKernelGGL((batch_spread_kernel<TD, TT, DYN, NEG, ROT && (RB > 1), RB>),
                            dim3(b->grid_spread), dim3(kSpreadBlock), spread_lds(RB, b->nt_max),
-                           st, v.probs, v.scales, v.sitems, (int)b->n_spread, v.ctr,
-                           reinterpret_cast<const TD *>(m), reinterpret_cast<TD *>(y));
+                           st, v.probs, v.scales, v.sitems, v.ctr + kSpreadQ,
+                           v.ctr_const + kQoff, reinterpret_cast<const TD *>(m),
+                           reinterpret_cast<TD *>(y));
         KB_LAUNCHED("batch spread");
         return KX_OK;
     } else {
@@ -539,8 +615,9 @@ template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
 int launch_bgather(const kx_batch *b, const View &v, const void *d, void *m, hipStream_t st)
 {
     hipLaunchKernelGGL((batch_gather_kernel<TD, TT, DYN, NEG, JW, P>), dim3(b->grid_gather),
-                       dim3(kGatherBlock), 0, st, v.probs, v.gitems, (int)b->n_gather, v.ctr + 2,
-                       reinterpret_cast<const TD *>(d), reinterpret_cast<TD *>(m));
+                       dim3(kGatherBlock), 0, st, v.probs, v.gitems, v.ctr + kGatherQ,
+                       v.ctr_const + kQoff + 16, reinterpret_cast<const TD *>(d),
+                       reinterpret_cast<TD *>(m));
     KB_LAUNCHED("batch gather");
     return KX_OK;
 }
@@ -642,12 +719,16 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
         nh_max = std::max(nh_max, a.nh);
     }
 
-    // work lists in order of decreasing cost (a stable sort keeps a problem's
-    // items in their natural order)
+    // work lists: spread groups = (receiver table, receiver block), gather
+    // groups = (source table, run of neighbouring tile groups), dealt to the
+    // eight XCD queues
     std::vector<SpreadItem> sitems;
     std::vector<int64_t> scost;
+    std::vector<int> sgroup;
     sitems.reserve(L.n_spread);
     scost.reserve(L.n_spread);
+    sgroup.reserve(L.n_spread);
+    std::map<std::pair<uintptr_t, int>, int> skeys;
     for (int p = 0; p < nprob; ++p) {
         const kx_plan &a = plans[p];
         for (int s = 0; s < a.ns; ++s)
@@ -656,36 +737,57 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
                 sitems.push_back({p, s, r0, 0});
                 // tile sweeps plus clearing and writing out the traces
                 scost.push_back(a.ntiles * kWave * nrl + (int64_t)nrl * a.nt * 4);
+                const auto key = std::make_pair(reinterpret_cast<uintptr_t>(a.off_r), r0);
+                auto it = skeys.emplace(key, (int)skeys.size()).first;
+                sgroup.push_back(it->second);
             }
     }
-    std::vector<int> order(sitems.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int i, int j) { return scost[i] > scost[j]; });
+    int sqoff[kXcds + 1];
+    const std::vector<int> sperm = deal(sgroup, (int)skeys.size(), scost, sqoff,
+                                        [&](int i, int j) {
+                                            const SpreadItem &u = sitems[i], &v = sitems[j];
+                                            return u.s != v.s ? u.s < v.s : u.p < v.p;
+                                        });
     std::vector<SpreadItem> ssorted(sitems.size());
-    for (size_t i = 0; i < order.size(); ++i)
-        ssorted[i] = sitems[order[i]];
+    for (size_t i = 0; i < sperm.size(); ++i)
+        ssorted[i] = sitems[sperm[i]];
 
     std::vector<GatherItem> gitems;
     std::vector<int64_t> gcost;
+    std::vector<int> ggroup;
     gitems.reserve(L.n_gather);
     gcost.reserve(L.n_gather);
+    ggroup.reserve(L.n_gather);
+    std::map<std::pair<uintptr_t, int>, int> gkeys;
     for (int p = 0; p < nprob; ++p) {
         const kx_plan &a = plans[p];
         const int ng = (int)((a.ntiles + NT - 1) / NT);
+        // neighbouring tiles read neighbouring trace windows: runs of tile
+        // groups, about one eighth of the image each
+        const int span = std::max(16, (ng + kXcds - 1) / kXcds);
         for (int g = 0; g < ng; ++g) {
             const int nti = std::min<int64_t>(NT, a.ntiles - (int64_t)g * NT);
             gitems.push_back({p, g});
             gcost.push_back((int64_t)nti * a.ns * a.nr);
+            const auto key = std::make_pair(reinterpret_cast<uintptr_t>(a.off_s), g / span);
+            auto it = gkeys.emplace(key, (int)gkeys.size()).first;
+            ggroup.push_back(it->second);
         }
     }
-    order.resize(gitems.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int i, int j) { return gcost[i] > gcost[j]; });
+    int gqoff[kXcds + 1];
+    const std::vector<int> gperm = deal(ggroup, (int)gkeys.size(), gcost, gqoff,
+                                        [&](int i, int j) {
+                                            const GatherItem &u = gitems[i], &v = gitems[j];
+                                            return u.L != v.L ? u.L < v.L : u.p < v.p;
+                                        });
     std::vector<GatherItem> gsorted(gitems.size());
-    for (size_t i = 0; i < order.size(); ++i)
-        gsorted[i] = gitems[order[i]];
+    for (size_t i = 0; i < gperm.size(); ++i)
+        gsorted[i] = gitems[gperm[i]];
+    std::vector<int> words(kCounters, 0);
+    for (int k = 0; k <= kXcds; ++k) {
+        words[kQoff + k] = sqoff[k];
+        words[kQoff + 16 + k] = gqoff[k];
+    }
 
     // persistent grids: the resident workgroups of the whole chip
     const Fns f = fns_of(q.dtype, q.tab_fmt);
@@ -725,6 +827,8 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
     KB_HIP(hipMemcpyAsync(base + L.gitems, gsorted.data(), sizeof(GatherItem) * gsorted.size(),
                           hipMemcpyHostToDevice, st));
     KB_HIP(hipMemcpyAsync(base + L.cprefix, cprefix.data(), sizeof(int64_t) * (nprob + 1),
+                          hipMemcpyHostToDevice, st));
+    KB_HIP(hipMemcpyAsync(base + L.ctr, words.data(), sizeof(int) * kCounters,
                           hipMemcpyHostToDevice, st));
     KB_HIP(hipStreamSynchronize(st));  // the host vectors end here
     return KX_OK;

@@ -12,6 +12,7 @@ synchronises every iteration and evaluates the reference's condition on
 the host.
 """
 import ctypes
+import gc
 import time
 
 import numpy as np
@@ -90,10 +91,20 @@ def _replay(graph, n, sc, active_index, device):
 
 
 def _capture(body, device):
+    """Capture ``body`` in a HIP graph.  Python's cyclic garbage collector is
+    kept off during the capture (torch collects once before it begins):
+    collecting an earlier solver there would destroy its graph while the
+    stream is capturing, which HIP forbids."""
     graph = torch.cuda.CUDAGraph()
     torch.cuda.synchronize(device)
-    with torch.cuda.graph(graph):
-        body()
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.graph(graph):
+            body()
+    finally:
+        if enabled:
+            gc.enable()
     return graph
 
 

@@ -428,13 +428,14 @@ class Kirchhoff(LinearOperator):
     # by column (LO:436-447); here the k columns run as a batch of k copies
     # of this operator, one persistent launch per kernel sharing the tables
     def _copies(self, k):
+        _ = self.plan  # stale tables are rebuilt first
+        key = (k, self.rblock, self.sorder, self.gsplit)
         cache = self.__dict__.get("_mm")
-        if cache is None or cache[0] != k or cache[1]._keep[0] is not self._dev_tables:
+        if cache is None or cache[0] != key or cache[1]._keep[0] is not self._dev_tables:
             from pylops_amd.waveeqprocessing.kirchhoffbatch import KirchhoffBatch
-            _ = self.plan  # stale tables are rebuilt first
             B = KirchhoffBatch([self] * k, rblock=self.rblock, sorder=self.sorder,
                                gsplit=self.gsplit)
-            object.__setattr__(self, "_mm", (k, B))
+            object.__setattr__(self, "_mm", (key, B))
         return self._mm[1]
 
     def _multi(self, X, adjoint):

@@ -221,7 +221,7 @@ def test_batch_rejects_mixed_table_formats():
 
 def test_autotune_single_operator(tmp_path):
     """The tuner returns a valid shape set, caches it, and the tuned
-    operator gives the default operator's results."""
+    operator gives the default operator's results (to rounding)."""
     import pylops_amd
     from pylops_amd.waveeqprocessing import autotune
     rng = np.random.default_rng(31)
@@ -237,7 +237,9 @@ def test_autotune_single_operator(tmp_path):
                               sorder=best["sorder"], gsplit=best["gsplit"])
     m = rng.standard_normal(op.shape[1]).astype(np.float32)
     d = rng.standard_normal(op.shape[0]).astype(np.float32)
-    np.testing.assert_array_equal(op @ m, op0 @ m)
+    # another tile splits traveltimes into other integer bases and offsets:
+    # equal to float32 rounding of the interpolation weights
+    assert G.relerr(op @ m, op0 @ m) < 1e-6
     assert G.relerr(op.H @ d, op0.H @ d) < 1e-6
 
 
@@ -253,8 +255,10 @@ def test_autotune_batch(tmp_path):
     B0 = KirchhoffBatch([pylops_amd.Kirchhoff(**kw) for kw in kws])
     m = rng.standard_normal(B.shape[1]).astype(np.float32)
     d = rng.standard_normal(B.shape[0]).astype(np.float32)
-    np.testing.assert_array_equal(B @ m, B0 @ m)
-    assert G.relerr(B.H @ d, B0.H @ d) < 1e-6
+    # float16 offsets from another tile's bases round differently: the
+    # float16-table bar
+    assert G.relerr(B @ m, B0 @ m) < 2e-3
+    assert G.relerr(B.H @ d, B0.H @ d) < 2e-3
 
 
 def test_launch_attributes_update_the_plan():

@@ -166,3 +166,71 @@ def test_solvers_vs_library(shape):
     ref = sp_lsqr(A, y, iter_lim=shape[1], atol=0, btol=0, conlim=0)
     np.testing.assert_allclose(res[0], ref[0], rtol=1e-6)
     np.testing.assert_allclose(res[0], xt, rtol=1e-4)
+
+
+# ---------------------------------------------------------------------------
+# ISTA / FISTA restatement (oracle.shrinkage and thresholds)
+# ---------------------------------------------------------------------------
+def _brute_prox(x, penalty):
+    """argmin_u 0.5 (u - x)^2 + penalty(u) on a fine grid."""
+    u = np.linspace(-3 * abs(x) - 1, 3 * abs(x) + 1, 400001)
+    return u[np.argmin(0.5 * (u - x) ** 2 + penalty(u))]
+
+
+@pytest.mark.parametrize("thresh", [0.1, 0.7, 2.0])
+def test_thresholds_are_proximal_maps(thresh):
+    """soft = prox of t|u|, hard = prox of t [u != 0] (cut at sqrt(2t)),
+    half = prox of (t/2) |u|^(1/2), i.e. argmin (u - x)^2 + t |u|^(1/2)
+    (Xu et al. 2012; cut at 54^(1/3)/4 t^(2/3)): each checked by brute-force
+    minimisation away from the jump."""
+    cases = {"soft": (O.soft_threshold, lambda u: thresh * np.abs(u), 0.0),
+             "hard": (O.hard_threshold, lambda u: thresh * (u != 0), np.sqrt(2 * thresh)),
+             "half": (O.half_threshold, lambda u: 0.5 * thresh * np.sqrt(np.abs(u)),
+                      54 ** (1 / 3) / 4 * thresh ** (2 / 3))}
+    for name, (f, pen, cut) in cases.items():
+        for x in (-3.1, -1.3, -0.4, 0.05, 0.6, 1.7, 4.2):
+            if cut and abs(abs(x) - cut) < 0.05 * cut:
+                continue  # the global minimiser jumps at the cut
+            got = f(np.array([x]), thresh)[0]
+            want = _brute_prox(x, pen)
+            assert abs(got - want) < 2e-4 * max(1, abs(x)), (name, x, got, want)
+
+
+def test_fista_ista_fixed_point_is_the_lasso_optimum():
+    """With alpha = 1/L, many iterations reach a point satisfying the
+    optimality conditions of 0.5|y - Ax|^2 + (eps/2)|x|_1 (the threshold is
+    eps*alpha/2): A^T r = (eps/2) sign(x) on the support, |A^T r| <= eps/2
+    off it; eps -> 0 gives the least-squares solution."""
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((31, 11))
+    y = rng.standard_normal(31)
+    Op = O.DenseOp(A)
+    L = np.linalg.eigvalsh(A.T @ A).max()
+    eps = 2.0
+    for fast in (True, False):
+        x, it, _ = O.shrinkage(Op, y, 1.0 / L, niter=20000, eps=eps, tol=1e-14, fast=fast)
+        g = A.T @ (y - A @ x)
+        on = x != 0
+        np.testing.assert_allclose(g[on], eps / 2 * np.sign(x[on]), atol=1e-7)
+        assert np.all(np.abs(g[~on]) <= eps / 2 + 1e-7)
+        xls, _, _ = O.shrinkage(Op, y, 1.0 / L, niter=20000, eps=0.0, tol=1e-15, fast=fast)
+        np.testing.assert_allclose(xls, np.linalg.lstsq(A, y, rcond=None)[0], atol=1e-8)
+
+
+@pytest.mark.parametrize("shape", [(11, 11), (31, 11), (21, 41)])
+def test_fista_ista_recovery_as_reference_tests(shape):
+    """pytests/test_sparsity.py:259-343: a 3-spike model recovered to one
+    decimal with eps=0.5, 2000 iterations, every threshold kind."""
+    np.random.seed(42)
+    A = np.random.randn(*shape)
+    x = np.zeros(shape[1])
+    x[shape[1] // 2] = 1
+    x[3] = 1
+    x[shape[1] - 4] = -1
+    y = A @ x
+    alpha = 1.0 / np.linalg.eigvalsh(A.T @ A).max()
+    for kind in ("hard", "soft", "half"):
+        for fast in (False, True):
+            xinv, _, _ = O.shrinkage(O.DenseOp(A), y, alpha, niter=2000, eps=0.5, tol=0,
+                                     threshkind=kind, fast=fast)
+            np.testing.assert_array_almost_equal(x, xinv, decimal=1)

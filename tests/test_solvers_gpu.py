@@ -209,3 +209,22 @@ def test_graph_replay_matches_eager(solver):
         assert g[2] == eager[2] == 12
         np.testing.assert_allclose(g[-1], eager[-1], rtol=1e-5)
         assert G.relerr(g[0].cpu().numpy(), eager[0].cpu().numpy()) < 1e-4
+
+
+def test_graph_capture_with_pending_solver_garbage():
+    """Solvers freed by the cyclic garbage collector hold HIP graphs; a
+    collection during a later capture would destroy one while the stream
+    is capturing.  With collections forced at every allocation, repeated
+    graph-replayed solves still run."""
+    import gc
+    from pylops_amd.optimization.basic import cgls, lsqr
+    Op = _kop("float32")
+    y = Op @ np.ones(Op.shape[1], np.float32)
+    old = gc.get_threshold()
+    gc.set_threshold(1, 1, 1)
+    try:
+        for _ in range(3):
+            cgls(Op, y, niter=4, tol=0.0, graph=True)
+            lsqr(Op, y, niter=4, atol=0.0, btol=0.0, graph=True)
+    finally:
+        gc.set_threshold(*old)
