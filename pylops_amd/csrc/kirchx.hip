@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD 
 }
 
 template <typename TD>
-__global__ __launch_bounds__(kBlock) void conv_kernel(const TD *__restrict__ x, TD *__restrict__ y,
+__global__ __launch_bounds__(kConvThreads) void conv_kernel(const TD *__restrict__ x, TD *__restrict__ y,
                                                       int nt, const TD *__restrict__ h, int nh,
                                                       int nseg)
 {
@@ -577,17 +577,17 @@ extern "C" int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const vo
     const int nseg = (nt + kConvOut - 1) / kConvOut;
     if (ntr * nseg > INT32_MAX)
         return kx_fail(KX_ESHAPE, "conv: too many traces");
-    const size_t lds = (size_t)(nh + kConvOut + nh - 1) * (dtype == KX_F64 ? 8 : 4);
+    const size_t lds = (size_t)conv_lds_elems(nh) * (dtype == KX_F64 ? 8 : 4);
     if (lds > (size_t)kLdsBudget)
         return kx_fail(KX_ELDS, "conv: filter of %d taps does not fit LDS", nh);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const dim3 grid((unsigned)(ntr * nseg));
     if (dtype == KX_F32)
-        hipLaunchKernelGGL(conv_kernel<float>, grid, dim3(kBlock), lds, st,
+        hipLaunchKernelGGL(conv_kernel<float>, grid, dim3(kConvThreads), lds, st,
                            reinterpret_cast<const float *>(x), reinterpret_cast<float *>(y), nt,
                            reinterpret_cast<const float *>(h), nh, nseg);
     else if (dtype == KX_F64)
-        hipLaunchKernelGGL(conv_kernel<double>, grid, dim3(kBlock), lds, st,
+        hipLaunchKernelGGL(conv_kernel<double>, grid, dim3(kConvThreads), lds, st,
                            reinterpret_cast<const double *>(x), reinterpret_cast<double *>(y), nt,
                            reinterpret_cast<const double *>(h), nh, nseg);
     else

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kGatherBlock) void batch_gather_kernel(
 // One convolution block (kConvOut samples of one trace) per workgroup; the
 // problem is found by bisection of the block prefix.
 template <typename TD>
-__global__ __launch_bounds__(kBlock) void batch_conv_kernel(const BatchProb *__restrict__ probs,
+__global__ __launch_bounds__(kConvThreads) void batch_conv_kernel(const BatchProb *__restrict__ probs,
                                                             const int64_t *__restrict__ cprefix,
                                                             int nprob, const TD *__restrict__ x,
                                                             TD *__restrict__ y, int adjoint)
@@ -416,7 +416,7 @@ int check_batch(const kx_plan *plans, int nprob)
         if (spread_lds(1, a.nt) > (size_t)kLdsBudget)
             return bfail(KX_ELDS, "batch: problem %d: nt=%d does not fit one trace in LDS", p,
                          a.nt);
-        const size_t clds = (size_t)(a.nh + kConvOut + a.nh - 1) * dsize(a.dtype);
+        const size_t clds = (size_t)conv_lds_elems(a.nh) * dsize(a.dtype);
         if (clds > (size_t)kLdsBudget)
             return bfail(KX_ELDS, "batch: problem %d: filter of %d taps does not fit LDS", p,
                          a.nh);
@@ -649,13 +649,13 @@ int launch_bgather_d(const kx_batch *b, const View &v, const void *d, void *m, h
 int launch_bconv(const kx_batch *b, const View &v, const void *x, void *y, int adjoint,
                  hipStream_t st)
 {
-    const size_t lds = (size_t)(b->nh_max + kConvOut + b->nh_max - 1) * dsize(b->dtype);
+    const size_t lds = (size_t)conv_lds_elems(b->nh_max) * dsize(b->dtype);
     if (b->dtype == KX_F64)
-        hipLaunchKernelGGL(batch_conv_kernel<double>, dim3((unsigned)b->n_conv), dim3(kBlock), lds,
+        hipLaunchKernelGGL(batch_conv_kernel<double>, dim3((unsigned)b->n_conv), dim3(kConvThreads), lds,
                            st, v.probs, v.cprefix, b->nprob, reinterpret_cast<const double *>(x),
                            reinterpret_cast<double *>(y), adjoint);
     else
-        hipLaunchKernelGGL(batch_conv_kernel<float>, dim3((unsigned)b->n_conv), dim3(kBlock), lds,
+        hipLaunchKernelGGL(batch_conv_kernel<float>, dim3((unsigned)b->n_conv), dim3(kConvThreads), lds,
                            st, v.probs, v.cprefix, b->nprob, reinterpret_cast<const float *>(x),
                            reinterpret_cast<float *>(y), adjoint);
     KB_LAUNCHED("batch conv");

@@ -600,34 +600,78 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
 // ---------------------------------------------------------------------------
 // "same" convolution along time, odd filter
 // ---------------------------------------------------------------------------
-constexpr int kConvOut = 256;
+// One workgroup of kConvThreads computes kConvOut consecutive outputs of one
+// trace, kConvPer adjacent outputs per thread.  The flipped filter and the
+// trace window (outputs plus the filter's halo) are staged in LDS, zero
+// padded to whole 4-element groups; a float thread reads both as float4
+// (consecutive lanes read consecutive 16 bytes: no bank conflicts) and keeps
+// the 8 window samples of two groups in registers, so 16 multiply-adds cost
+// two LDS reads.
+constexpr int kConvThreads = 128;
+constexpr int kConvPer = 4;
+constexpr int kConvOut = kConvThreads * kConvPer;
 
-// Output samples [n0, n0 + kConvOut) of trace tr.
+// filter length padded to a whole number of 4-element groups
+__host__ __device__ __forceinline__ int conv_taps_padded(int nh) { return (nh + 3) & ~3; }
+
+// LDS elements of one convolution workgroup: filter + window + one group of
+// read-ahead
+__host__ __device__ __forceinline__ int conv_lds_elems(int nh)
+{
+    return conv_taps_padded(nh) + kConvOut + conv_taps_padded(nh) + 4;
+}
+
+// Output samples [n0, n0 + kConvOut) of trace tr:
+// y[n] = sum_k h[k] x[n + (nh - 1) / 2 - k].
 template <typename TD>
 __device__ __forceinline__ void conv_segment(const TD *__restrict__ x, TD *__restrict__ y, int nt,
                                              const TD *__restrict__ h, int nh, int64_t tr, int n0)
 {
     extern __shared__ __align__(16) unsigned char kx_smem[];
+    const int nhp = conv_taps_padded(nh);
     TD *hs = reinterpret_cast<TD *>(kx_smem);
-    TD *win = hs + nh;
+    TD *win = hs + nhp;
     const int c = (nh - 1) / 2;
     const int start = n0 + c - (nh - 1);
-    const int nwin = kConvOut + nh - 1;
+    const int nwin = kConvOut + nhp + 4;
     const TD *xr = x + tr * nt;
-    for (int k = threadIdx.x; k < nh; k += blockDim.x)
-        hs[k] = h[nh - 1 - k];  // flipped: out[i] = sum_k hs[k] win[i + k]
+    for (int k = threadIdx.x; k < nhp; k += blockDim.x)
+        hs[k] = k < nh ? h[nh - 1 - k] : TD(0);  // flipped: out[i] = sum_k hs[k] win[i + k]
     for (int j = threadIdx.x; j < nwin; j += blockDim.x) {
         const int n = start + j;
         win[j] = (n >= 0 && n < nt) ? xr[n] : TD(0);
     }
     __syncthreads();
-    const int i = threadIdx.x;
-    if (i < kConvOut && n0 + i < nt) {
-        TD acc = TD(0);
-        for (int k = 0; k < nh; ++k)
-            acc += hs[k] * win[i + k];
-        y[tr * nt + n0 + i] = acc;
+    const int i0 = threadIdx.x * kConvPer;  // first output of this thread
+    TD acc[kConvPer] = {};
+    if constexpr (sizeof(TD) == 4) {
+        const float4 *w4 = reinterpret_cast<const float4 *>(win) + threadIdx.x;
+        const float4 *h4 = reinterpret_cast<const float4 *>(hs);
+        float4 cur = w4[0];
+        for (int g = 0; g < nhp / 4; ++g) {
+            const float4 nxt = w4[g + 1];
+            const float4 hq = h4[g];
+            const float w[8] = {cur.x, cur.y, cur.z, cur.w, nxt.x, nxt.y, nxt.z, nxt.w};
+            const float hh[4] = {hq.x, hq.y, hq.z, hq.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < kConvPer; ++r)
+                    acc[r] += hh[q] * w[q + r];
+            cur = nxt;
+        }
+    } else {
+        for (int k = 0; k < nhp; ++k) {
+            const TD hk = hs[k];
+#pragma unroll
+            for (int r = 0; r < kConvPer; ++r)
+                acc[r] += hk * win[i0 + r + k];
+        }
     }
+#pragma unroll
+    for (int r = 0; r < kConvPer; ++r)
+        if (n0 + i0 + r < nt)
+            y[tr * nt + n0 + i0 + r] = acc[r];
     __syncthreads();  // the window is read before a next item loads its own
 }
 

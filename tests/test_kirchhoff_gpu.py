@@ -185,6 +185,31 @@ def test_conv_matches_oracle():
         assert G.relerr(y.cpu().numpy(), O.conv_same(x, hp)) < 1e-6
 
 
+@pytest.mark.parametrize("nt", [3, 300, 512, 513, 1501])
+@pytest.mark.parametrize("nh", [1, 3, 81, 601, 1023])
+def test_conv_lengths(nt, nh):
+    """The convolution kernel against numpy's direct convolution for traces
+    shorter and longer than one workgroup's 512 outputs and filters shorter
+    and longer than them (float32 and float64)."""
+    import torch
+    from pylops_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(nt * 7 + nh)
+    x = rng.standard_normal((5, nt))
+    h = rng.standard_normal(nh)
+    ref = O.conv_same(x, h)
+    for dt, kx, tol in ((torch.float32, _lib.KX_F32, 1e-6), (torch.float64, _lib.KX_F64, 1e-13)):
+        xt = torch.as_tensor(x, dtype=dt, device="cuda")
+        ht = torch.as_tensor(h, dtype=dt, device="cuda")
+        yt = torch.full_like(xt, float("nan"))
+        rc = L.kx_conv(_lib.ptr(xt), _lib.ptr(yt), 5, nt, _lib.ptr(ht), nh, kx,
+                       _lib.stream_ptr())
+        _lib.check(rc, "kx_conv")
+        y = yt.cpu().numpy()
+        assert np.all(np.isfinite(y))
+        assert G.relerr(y, ref) < tol, (nt, nh, dt)
+
+
 def test_dot_fp64_accumulation():
     import torch
     from pylops_amd import _lib
