@@ -209,28 +209,41 @@ __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c,
     const int iz = c.tzi * a.tz + lz, ix = c.txi * a.tx + lx, iy = c.tyi * a.ty + ly;
     const bool valid = (iz < a.nz) & (ix < a.nx) & (iy < a.ny);
     const int orig = (c.tyi * a.ty * a.nx + c.txi * a.tx) * a.nz + c.tzi * a.tz;
-    L.mv = valid ? m[orig + rel] : TD(0);
+    // Row pointers are wave-uniform and the lane offsets 32-bit unsigned, so
+    // every load takes the scalar-base form (one VALU op of addressing).
+    L.mv = valid ? (m + orig)[(unsigned)rel] : TD(0);
     const size_t ts = (size_t)t * a.ns + s;        // (tile, source) row
     const size_t tr = (size_t)t * a.nr + r0;       // (tile, first receiver) row
-    L.os = reinterpret_cast<const TT *>(a.off_s)[ts * kWave + lane];
+    L.os = (reinterpret_cast<const TT *>(a.off_s) + ts * kWave)[(unsigned)lane];
     L.bs = a.base_s[ts + vz];
     if constexpr (DYN) {
-        L.ps = reinterpret_cast<const TD *>(a.wp_s)[ts * kWave + lane];
-        L.qs = reinterpret_cast<const TD *>(a.wq_s)[ts * kWave + lane];
+        L.ps = (reinterpret_cast<const TD *>(a.wp_s) + ts * kWave)[(unsigned)lane];
+        L.qs = (reinterpret_cast<const TD *>(a.wq_s) + ts * kWave)[(unsigned)lane];
     }
+    const TT *orow = reinterpret_cast<const TT *>(a.off_r) + tr * kWave;
+    const int *brow = a.base_r + tr;
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         int j = step_receiver<RB, ROT>(lane, r);
         j = j < nrl ? j : 0;  // rows past the last receiver read row 0
-        const size_t e = (tr + j) * kWave + lane;
-        L.orr[r] = reinterpret_cast<const TT *>(a.off_r)[e];
-        L.br[r] = a.base_r[tr + j];
+        const unsigned e = (unsigned)(j * kWave + lane);
+        L.orr[r] = orow[e];
+        L.br[r] = brow[(unsigned)(j + vz)];
         if constexpr (DYN) {
-            L.pr[r] = reinterpret_cast<const TD *>(a.wp_r)[e];
-            L.qr[r] = reinterpret_cast<const TD *>(a.wq_r)[e];
+            L.pr[r] = (reinterpret_cast<const TD *>(a.wp_r) + tr * kWave)[e];
+            L.qr[r] = (reinterpret_cast<const TD *>(a.wq_r) + tr * kWave)[e];
         }
     }
 }
+
+// floor(x + 1/2) as an integer, one conversion instruction (|x| < 2^31)
+__device__ __forceinline__ int round_half_up(float x)
+{
+    int i;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
+__device__ __forceinline__ int round_half_up(double x) { return (int)floor(x + 0.5); }
 
 template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
 __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
@@ -306,12 +319,15 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
                 }
                 if (lrow[r] >= 0 && (unsigned)k < ntm1 && w != TD(0)) {
                     const int it = lrow[r] + k;
-                    const TD w0 = w * (TD)(TC(1) - f);
                     const TD w1 = w * (TD)f;
                     if constexpr (FIX) {
-                        atomicAdd(&acci[it], (unsigned long long)(long long)__float2int_rn(w0));
-                        atomicAdd(&acci[it + 1], (unsigned long long)(long long)__float2int_rn(w1));
+                        // w (1 - f) in one rounding; fixed point by
+                        // floor(x + 1/2) in one conversion
+                        const TD w0 = fma(-w, (TD)f, w);
+                        atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
+                        atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
                     } else {
+                        const TD w0 = w * (TD)(TC(1) - f);
                         atomicAdd(&accd[it], (double)w0);
                         atomicAdd(&accd[it + 1], (double)w1);
                     }
@@ -501,42 +517,65 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
             }
 
             // the window of source i in round r starts at bs + br(r) - kWin / 2;
-            // lane l stages its sample l
+            // lane l stages its sample l (a clamped load, flagged in svok when
+            // the sample lies inside the trace; outside ones become zero when
+            // the round begins, so that the loads are waited for only then);
+            // the aperture gate of the round is loaded with it
             TD sv[JW];
+            bool svok[JW];
+            int izn[JW] = {};
 #define KX_STAGE(R, BR)                                                                        \
     _Pragma("unroll") for (int i = 0; i < JW; ++i)                                              \
     {                                                                                          \
         const int e = bs[i] + (BR) - kWin / 2 + lane;                                          \
-        sv[i] = (unsigned)e < (unsigned)a.nt ? dblk[soff[i] + (R) * a.nt + e] : TD(0);          \
+        const int ec = min(max(e, 0), a.nt - 1);                                               \
+        sv[i] = dblk[(unsigned)(soff[i] + (R) * a.nt + ec)];                                   \
+        svok[i] = (unsigned)e < (unsigned)a.nt;                                                \
+        if constexpr (DYN)                                                                     \
+            izn[i] = a.izmin[(unsigned)(izoff[i] + (R))];                                      \
     }
-            // receiver side of round r, and receiver base of round r + 1
-            TT orr_c = offr[((size_t)t * a.nr) * kWave + lane];
-            int br_c = a.base_r[(size_t)t * a.nr + vz];
+            // receiver side of the round ahead; receiver bases two rounds
+            // ahead (the window addresses of the round ahead depend on them)
+            const size_t t0r = (size_t)t * a.nr;
+            TT orr_c = offr[t0r * kWave + lane];
+            int br_c = a.base_r[t0r + vz];
+            int br_n = a.nr > 1 ? a.base_r[t0r + 1 + vz] : 0;
             TD pr_c = TD(0), qr_c = TD(0);
             if constexpr (DYN) {
-                pr_c = wpr[((size_t)t * a.nr) * kWave + lane];
-                qr_c = wqr[((size_t)t * a.nr) * kWave + lane];
+                pr_c = wpr[t0r * kWave + lane];
+                qr_c = wqr[t0r * kWave + lane];
             }
             KX_STAGE(0, br_c)
             for (int r = 0; r < a.nr; ++r) {
                 // this round's samples into the wave's LDS region
+                int izr[JW];
 #pragma unroll
-                for (int i = 0; i < JW; ++i)
-                    w[i][lane] = sv[i];
+                for (int i = 0; i < JW; ++i) {
+                    w[i][lane] = svok[i] ? sv[i] : TD(0);
+                    izr[i] = izn[i];
+                }
                 const TC orr = (TC)ld_off(&orr_c);
                 const int br = br_c;
                 const TD pr = pr_c, qr = qr_c;
                 // next round's receiver side and samples, in flight meanwhile
                 if (r + 1 < a.nr) {
-                    const size_t ro = (size_t)t * a.nr + r + 1;
+                    const size_t ro = t0r + r + 1;
                     orr_c = offr[ro * kWave + lane];
-                    br_c = a.base_r[ro + vz];
                     if constexpr (DYN) {
                         pr_c = wpr[ro * kWave + lane];
                         qr_c = wqr[ro * kWave + lane];
                     }
+                    br_c = br_n;
                     KX_STAGE(r + 1, br_c)
+                    if (r + 2 < a.nr)
+                        br_n = a.base_r[ro + 1 + vz];
                 }
+                // Each lane reads its two samples from the staged windows.  A
+                // sample outside its window (rare: the tile's own offsets span
+                // a few tens of samples) is marked, and the wave fixes the
+                // round's sum afterwards from global memory, so that the
+                // common path has no per-source branch.
+                unsigned miss = 0;
 #pragma unroll
                 for (int i = 0; i < JW; ++i) {
                     const TC T = os[i] + orr;
@@ -551,28 +590,52 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                             f -= TC(1);
                         }
                     }
-                    // The LDS read is unconditional (clamped index); a sample
-                    // outside the window overwrites it from global memory (a
-                    // branch between the two loads would merge them into one
-                    // generic-address load).
                     const int kk = it - (bs[i] + br - kWin / 2);
                     const int kc = min(max(kk, 0), kWin - 2);
-                    TD q0 = w[i][kc];
-                    TD q1 = w[i][kc + 1];
+                    const TD q0 = w[i][kc];
+                    const TD q1 = w[i][kc + 1];
                     const bool in_trace = (unsigned)it < ntm1;
-                    if (in_trace && kk != kc) {
-                        const TD *q = dblk + (soff[i] + r * a.nt + it);
-                        q0 = q[0];
-                        q1 = q[1];
-                    }
-                    TD v = q0 * (TD)(TC(1) - f) + q1 * (TD)f;
+                    miss |= (in_trace && kk != kc) ? (1u << i) : 0u;
+                    TD v = fma((TD)f, q1 - q0, q0);
                     if constexpr (DYN) {
                         TD wgt = ps[i] * pr + qs[i] * qr;
-                        if (p.iz < a.izmin[izoff[i] + r])
+                        if (p.iz < izr[i])
                             wgt = TD(0);
                         v *= wgt;
                     }
                     acc += in_trace ? v : TD(0);
+                }
+                if (__ballot(miss != 0) != 0) {
+                    // replace the window values of the marked sources by the
+                    // samples themselves
+#pragma unroll
+                    for (int i = 0; i < JW; ++i) {
+                        if ((miss >> i) & 1u) {
+                            const TC T = os[i] + orr;
+                            const TC fl = floor(T);
+                            TC f = T - fl;
+                            int it = (int)fl + bs[i] + br;
+                            if constexpr (NEG) {
+                                if (it == -1 && f > TC(0)) {
+                                    it = 0;
+                                    f -= TC(1);
+                                }
+                            }
+                            const int kk = it - (bs[i] + br - kWin / 2);
+                            const int kc = min(max(kk, 0), kWin - 2);
+                            const TD *q = dblk + (soff[i] + r * a.nt + it);
+                            const TD g0 = q[0], g1 = q[1];
+                            const TD l0 = w[i][kc], l1 = w[i][kc + 1];
+                            TD dv = fma((TD)f, g1 - g0, g0) - fma((TD)f, l1 - l0, l0);
+                            if constexpr (DYN) {
+                                TD wgt = ps[i] * pr + qs[i] * qr;
+                                if (p.iz < izr[i])
+                                    wgt = TD(0);
+                                dv *= wgt;
+                            }
+                            acc += dv;
+                        }
+                    }
                 }
             }
 #undef KX_STAGE
