@@ -238,9 +238,11 @@ def main():
     ap.add_argument("--tables", default="f32", choices=["f32", "f16"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true",
-                    help="also time the 3D workload with float32 and float16 tables")
+                    help="also the 3D precision report (float32 and float16 tables vs float64, "
+                         "fwd, adj and 20 CGLS iterations)")
     ap.add_argument("--no-solve", action="store_true",
-                    help="skip timing of the 30-iteration CGLS / LSQR solves")
+                    help="skip the 30-iteration CGLS / LSQR timings and the 2D precision "
+                         "report (both run solvers)")
     ap.add_argument("--adam-steps", type=int, default=50,
                     help="Adam steps of the autograd workload")
     ap.add_argument("--no-batch", action="store_true",
@@ -360,8 +362,10 @@ def main():
     }
     if not args.no_solve:
         out["solve"] = time_solvers(Op, dobs)
+    if not args.no_solve:
+        out["precision"] = precision_report(dev, "2d", niter=10)
     if args.extra:
-        out["extra"] = extra_3d(dev)
+        out["extra"] = precision_report(dev, "3d", niter=20)
     if not args.no_batch:
         out["batch"] = time_batch(dev, sequential=not args.no_solve)[0]
     if not args.no_cpu:
@@ -709,38 +713,58 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
     return out
 
 
-def extra_3d(dev):
-    """3D workload: fp32 vs fp16 tables, forward+adjoint time and error of
-    the fp16 result against the fp32 result."""
+def precision_report(dev, which, niter):
+    """Errors of the float32 operator with float32 and float16 tables
+    against the float64 operator (float64 tables, float64 arithmetic; its
+    kernels are checked against the oracle to 1e-10), for the forward of the
+    reflectivity, the adjoint of the float64 data and the model after
+    `niter` CGLS iterations; and the fwd+adj time of both table formats."""
     import torch
     import pylops_amd
-    rng = np.random.default_rng(1003)
-    kw, refl, desc = workload_3d(rng, dev)
-    res = {}
-    m = torch.as_tensor(refl.astype(np.float32).ravel(), device=dev)
-    outs = {}
+    from pylops_amd.optimization.basic import cgls
+    rng = np.random.default_rng(1003 if which == "3d" else 1002)
+    kw, refl, desc = workload_3d(rng, dev) if which == "3d" else workload_2d(rng)
+    Op64 = pylops_amd.Kirchhoff(dtype="float64", **kw)
+    m64 = torch.as_tensor(np.asarray(refl, dtype=np.float64).ravel(), device=dev)
+    d64 = Op64 @ m64
+    a64 = Op64.H @ d64
+    r64 = cgls(Op64, d64, niter=niter, tol=0.0, graph=True)
+    x64, cost64 = r64[0].reshape(-1), np.asarray(r64[5])
+    del Op64
+    torch.cuda.empty_cache()
+
+    def rel(u, v):
+        return float((u.double().reshape(-1) - v.reshape(-1)).norm() / v.norm())
+
+    res = {"workload": desc["workload"], "reference": "float64 operator", "cgls_niter": niter}
     for tabs in ("f32", "f16"):
         Op = pylops_amd.Kirchhoff(dtype="float32", tabledtype="float16" if tabs == "f16" else None,
                                   **kw)
+        m = m64.float()
+        d32 = d64.float()
         d = Op @ m
-        a = Op.H @ d
+        a = Op.H @ d32
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         n = 3
         for _ in range(n):
-            d = Op @ m
-            a = Op.H @ d
+            Op @ m
+            Op.H @ d32
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / n
-        outs[tabs] = (d, a)
+        rx = cgls(Op, d32, niter=niter, tol=0.0, graph=True)
+        x, cost = rx[0].reshape(-1), np.asarray(rx[5])
         res[tabs] = {"ms_fwd_adj": round(dt * 1e3, 3),
-                     "gsamples_s": round(2 * Op.ni * Op.ns * Op.nr / dt / 1e9, 2)}
+                     "gsamples_s": round(2 * Op.ni * Op.ns * Op.nr / dt / 1e9, 2),
+                     "rel_err_fwd": rel(d, d64), "rel_err_adj": rel(a, a64),
+                     "rel_err_cgls_x": rel(x, x64),
+                     "cgls_residual_norm_rel_diff_max": float(
+                         np.max(np.abs(cost - cost64) / cost64)),
+                     "cgls_residual_norm_reduction": float(cost[-1] / cost[0]),
+                     "cgls_residual_norm_reduction_fp64": float(cost64[-1] / cost64[0])}
         del Op
-    d32, a32 = outs["f32"]
-    d16, a16 = outs["f16"]
-    res["fp16_rel_err_fwd_vs_fp32"] = float((d16 - d32).norm() / d32.norm())
+        torch.cuda.empty_cache()
     res["fp16_speedup"] = round(res["f32"]["ms_fwd_adj"] / res["f16"]["ms_fwd_adj"], 3)
-    res["workload"] = desc["workload"]
     return res
 
 
