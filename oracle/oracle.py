@@ -15,6 +15,9 @@ Contents (reference file:line each part follows):
   same loop nests as K 683-1001), orchestration K 1048-1143
 * ``conv_same`` -- Convolve1D short-filter path, convolve1d.py 57-115
 * ``cgls`` / ``lsqr`` -- pylops/optimization/cls_basic.py 418-708, 835-1351
+* ``eikonal_fsm`` -- traveltime tables of the eikonal mode, K 609-638
+  (the reference calls scikit-fmm, absent here: restated as fast sweeping
+  from skfmm's zero contour, pinned by closed forms)
 * ``ista`` / ``fista`` and their thresholds -- pylops/optimization/
   cls_sparsity.py 36-200, 1313-2160 (synthesis form, no SOp)
 * ``dottest`` -- pylops/utils/dottest.py 10-132
@@ -592,6 +595,117 @@ def lsqr(Op, y, x0=None, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8,
         iiter += 1
     return (x, istop, iiter, r1norm, r2norm, anorm, acond, arnorm, xnorm, var,
             np.array(cost))
+
+
+_EIK_BIG = 1e30
+_EIK_R = 2
+
+
+def _godunov(ax, ay, az, hx, hy, hz, sl, three):
+    """Largest t with sum_k ((t - a_k)/h_k)^2 = s^2 over the axes below t
+    (first-order Godunov upwind), vectorised; same operation order as the
+    device kernel."""
+    a0, a1, a2 = ax.copy(), az.copy(), ay.copy()
+    h0 = np.full_like(a0, hx)
+    h1 = np.full_like(a0, hz)
+    h2 = np.full_like(a0, hy)
+
+    def swap(u, v, hu, hv):
+        m = v < u
+        return (np.where(m, v, u), np.where(m, u, v), np.where(m, hv, hu), np.where(m, hu, hv))
+
+    a0, a1, h0, h1 = swap(a0, a1, h0, h1)
+    if three:
+        a1, a2, h1, h2 = swap(a1, a2, h1, h2)
+        a0, a1, h0, h1 = swap(a0, a1, h0, h1)
+    t1 = a0 + sl * h0
+    w0 = 1.0 / (h0 * h0)
+    w1 = 1.0 / (h1 * h1)
+    A = w0 + w1
+    B = a0 * w0 + a1 * w1
+    C = a0 * a0 * w0 + a1 * a1 * w1 - sl * sl
+    t2 = (B + np.sqrt(np.maximum(B * B - A * C, 0.0))) / A
+    m2 = t1 > a1
+    t = np.where(m2, t2, t1)
+    if three:
+        w2 = 1.0 / (h2 * h2)
+        A3 = A + w2
+        B3 = B + a2 * w2
+        C3 = C + a2 * a2 * w2
+        t3 = (B3 + np.sqrt(np.maximum(B3 * B3 - A3 * C3, 0.0))) / A3
+        t = np.where(m2 & (t2 > a2), t3, t)
+    return t
+
+
+def eikonal_fsm(slow, src, h, max_iter=20):
+    """First-arrival times T[n][ny, nx, nz] from the zero contour of phi = -1
+    at each source cell src[s] = (ix, iy, iz) and +1 elsewhere, for the
+    slowness slow[ny, nx, nz] (2D: ny = 1, hy unused), by fast sweeping with
+    first-order Godunov updates.  Cells within two cells (per axis) of the
+    source hold the straight-ray time from the contour (a diamond through
+    the half-way points to the neighbouring cell centres) and stay fixed.
+    Returns (T, rounds)."""
+    slow = np.asarray(slow, dtype=np.float64)
+    ny, nx, nz = slow.shape
+    hx, hy, hz = h
+    three = ny > 1
+    src = np.asarray(src, dtype=np.int64).reshape(-1, 3)
+    n = len(src)
+    IY, IX, IZ = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+    T = np.full((n, ny, nx, nz), _EIK_BIG)
+    frozen = np.zeros((n, ny, nx, nz), dtype=bool)
+    for k, (sx, sy, sz) in enumerate(src):
+        box = ((np.abs(IX - sx) <= _EIK_R) & (np.abs(IY - sy) <= _EIK_R)
+               & (np.abs(IZ - sz) <= _EIK_R))
+        dx = (IX - sx) * hx
+        dy = (IY - sy) * (hy if three else 1.0)
+        dz = (IZ - sz) * hz
+        rho = np.sqrt(dx * dx + dy * dy + dz * dz)
+        S = 2.0 * (np.abs(dx) / hx + np.abs(dz) / hz + (np.abs(dy) / hy if three else 0.0))
+        sl = 0.5 * (slow + slow[sy, sx, sz])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            v = np.where(S > 0, sl * rho * np.maximum(0.0, 1.0 - 1.0 / S), 0.0)
+        T[k][box] = v[box]
+        frozen[k] = box
+    ndiag = (nx - 1) + (ny - 1) + (nz - 1) + 1
+    ndir = 8 if three else 4
+    rounds = 0
+    for _ in range(max_iter):
+        rounds += 1
+        changed = False
+        for q in range(ndir):
+            fx = (q & 1) != 0
+            fz = (q & 2) != 0
+            fy = (q & 4) != 0
+            PX = np.where(fx, nx - 1 - IX, IX)
+            PY = np.where(fy, ny - 1 - IY, IY)
+            PZ = np.where(fz, nz - 1 - IZ, IZ)
+            diag = PX + PY + PZ
+            for d in range(ndiag):
+                cy, cx, cz = np.nonzero(diag == d)
+                if len(cx) == 0:
+                    continue
+                Tc = T[:, cy, cx, cz]
+                BIG = _EIK_BIG
+                ax = np.minimum(np.where(cx > 0, T[:, cy, np.maximum(cx - 1, 0), cz], BIG),
+                                np.where(cx < nx - 1, T[:, cy, np.minimum(cx + 1, nx - 1), cz], BIG))
+                az = np.minimum(np.where(cz > 0, T[:, cy, cx, np.maximum(cz - 1, 0)], BIG),
+                                np.where(cz < nz - 1, T[:, cy, cx, np.minimum(cz + 1, nz - 1)], BIG))
+                if three:
+                    ay = np.minimum(
+                        np.where(cy > 0, T[:, np.maximum(cy - 1, 0), cx, cz], BIG),
+                        np.where(cy < ny - 1, T[:, np.minimum(cy + 1, ny - 1), cx, cz], BIG))
+                else:
+                    ay = np.full_like(ax, BIG)
+                t = _godunov(ax, ay, az, hx, hy if three else 1.0, hz,
+                             np.broadcast_to(slow[cy, cx, cz], Tc.shape), three)
+                upd = (t < Tc) & ~frozen[:, cy, cx, cz]
+                if np.any(upd & (Tc - t > 1e-13 * t)):
+                    changed = True
+                T[:, cy, cx, cz] = np.where(upd, t, Tc)
+        if not changed:
+            break
+    return T, rounds
 
 
 def soft_threshold(x, thresh):

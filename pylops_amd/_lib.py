@@ -26,6 +26,7 @@ EXPORTS = (
     "kx_cgls_scalar", "kx_lsqr_scalar", "kx_batch_dev_bytes", "kx_batch_build",
     "kx_batch_workspace_bytes", "kx_batch_forward", "kx_batch_adjoint",
     "kx_seg_workspace_bytes", "kx_seg_vec", "kx_seg_dot", "kx_cgls_scalar_batch",
+    "kx_eikonal",
 )
 
 # solver scalar layouts and vector ops (include/kirchx.h)
@@ -205,6 +206,8 @@ def _declare(L):
         "kx_seg_vec": ([I32, PS, I32, P, P, P, I32, I32, D, I32, I32, P, P], ctypes.c_int),
         "kx_seg_dot": ([PS, I32, P, P, P, I32, I32, I32, P, P], ctypes.c_int),
         "kx_cgls_scalar_batch": ([P, I32, P, P, I32, I32, P], ctypes.c_int),
+        "kx_eikonal": ([I32, I32, I32, I32, D, D, D, P, I32, P, P, I32, P,
+                        ctypes.POINTER(I32), P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -117,8 +117,8 @@ __device__ __forceinline__ int64_t point_index(const KArgs &a, const TilePoint &
 // forward: spread into LDS-resident traces
 // ---------------------------------------------------------------------------
 // LDS accumulation.  For float data every contribution c (rounded to float
-// as the reference rounds it) is added as the integer rint(c * 2^e) to a
-// 64-bit fixed-point sample, with one power-of-two scale 2^e per launch
+// as the reference rounds it) is added as the integer floor(c * 2^e + 1/2)
+// to a 64-bit fixed-point sample, with one power-of-two scale 2^e per launch
 // chosen from max|m| so that |c| 2^e <= 2^30: the rounding of a contribution
 // is at most 2^-31 of the largest possible one, the sum itself is exact and
 // does not depend on the order of the adds, and it cannot overflow (at most

@@ -67,6 +67,63 @@ def analytic(z, x, srcs, recs, vel, y, dims, dsamp, device):
     return ts, tr, ds, dr, gradient(ts, dims, dsamp), gradient(tr, dims, dsamp)
 
 
+def eikonal(pts, vel, origin, dsamp, dims, device, unit=False, max_iter=20):
+    """Traveltime (or, with ``unit``, distance) tables of shape (ni, n) from
+    the grid cells nearest to the points ``pts`` ((x, z) or (y, x, z) rows,
+    rounded as K:617-633 rounds them), by the device eikonal solver
+    (``kx_eikonal``).  ``vel`` is a scalar or an array of the grid's
+    shape."""
+    import ctypes
+    ndims = len(dims)
+    pts = np.asarray(pts, dtype=np.float64)
+    cells = np.round((pts.T - np.asarray(origin)) / np.asarray(dsamp)).astype(np.int32)
+    if np.any(cells < 0) or np.any(cells >= np.asarray(dims)):
+        raise ValueError("every source and receiver must lie inside the image grid "
+                         "for mode='eikonal'")
+    if ndims == 2:
+        (nx, nz), ny = dims, 1
+        hx, hz, hy = float(dsamp[0]), float(dsamp[1]), 1.0
+        src = np.stack((cells[:, 0], np.zeros(len(cells), np.int32), cells[:, 1]), axis=1)
+    else:
+        ny, nx, nz = dims
+        hy, hx, hz = (float(v) for v in dsamp)
+        src = np.stack((cells[:, 1], cells[:, 0], cells[:, 2]), axis=1)
+    ni = ny * nx * nz
+    if unit:
+        slow = torch.ones(ni, dtype=torch.float64, device=device)
+    else:
+        v = as_f64(vel, device)
+        v = v.expand(ni) if v.numel() == 1 else v.reshape(-1)
+        if v.numel() != ni:
+            raise ValueError(f"vel of {v.numel()} values for a grid of {ni} points")
+        slow = 1.0 / v
+    n = len(src)
+    srcd = torch.as_tensor(np.ascontiguousarray(src, dtype=np.int32), device=device)
+    out = torch.empty((n, ni), dtype=torch.float64, device=device)
+    flag = torch.zeros(1, dtype=torch.int32, device=device)
+    iters = ctypes.c_int32(0)
+    L = _lib.load()
+    rc = L.kx_eikonal(ndims, nx, ny, nz, hx, hy, hz, _lib.ptr(slow.contiguous()), n,
+                      _lib.ptr(srcd), _lib.ptr(out), max_iter, _lib.ptr(flag),
+                      ctypes.byref(iters), _lib.stream_ptr(device))
+    _lib.check(rc, "kx_eikonal")
+    return out.T
+
+
+def eikonal_tables(srcs, recs, vel, origin, dsamp, dims, device, distances):
+    """(trav_srcs, trav_recs, dist_srcs, dist_recs, grad_srcs, grad_recs) of
+    the eikonal mode (K:609-657): first-arrival times from each source and
+    receiver cell, the unit-speed distances when ``distances`` (dynamic
+    amplitudes need them), and the traveltime gradients."""
+    ts = eikonal(srcs, vel, origin, dsamp, dims, device)
+    tr = eikonal(recs, vel, origin, dsamp, dims, device)
+    ds = dr = None
+    if distances:
+        ds = eikonal(srcs, None, origin, dsamp, dims, device, unit=True)
+        dr = eikonal(recs, None, origin, dsamp, dims, device, unit=True)
+    return ts, tr, ds, dr, gradient(ts, dims, dsamp), gradient(tr, dims, dsamp)
+
+
 def gradient(tab, dims, dsamp=None):
     """np.gradient(tab.reshape(*dims, n), *dsamp, axis=range(ndims)) with
     edge_order=1; dsamp=None means unit spacing."""

@@ -21,6 +21,7 @@ device name.
 import ctypes
 import json
 import os
+import warnings
 
 import numpy as np
 import torch
@@ -54,13 +55,18 @@ def _load(path):
 
 
 def _store(path, key, value):
+    """Adds one result to the cache file; a cache that cannot be written
+    (read-only home, full disk) is skipped with a warning."""
     d = _load(path)
     d[key] = value
-    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    tmp = path + ".tmp"
-    with open(tmp, "w") as f:
-        json.dump(d, f, indent=1, sort_keys=True)
-    os.replace(tmp, path)
+    try:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError as e:
+        warnings.warn(f"autotune cache {path} not written: {e}")
 
 
 def _geometry(kw):

@@ -112,7 +112,7 @@ class Kirchhoff(LinearOperator):
         dev = self.device
 
         (self.ndims, _, dims, self.ny, self.nx, self.nz, ns, nr, _, _, _, dsamp,
-         _) = Kirchhoff._identify_geometry(z, x, srcs, recs, y=y)
+         origin) = Kirchhoff._identify_geometry(z, x, srcs, recs, y=y)
         self.dt = t[1] - t[0]
         self.nt = len(t)
         self.dynamic = dynamic
@@ -123,11 +123,14 @@ class Kirchhoff(LinearOperator):
         ni = int(np.prod(dims))
         if mode in ("analytic", "eikonal"):
             if mode == "eikonal":
-                raise NotImplementedError(
-                    "mode='eikonal' needs an eikonal solver, which is not provided")
-            if not isinstance(vel, (float, int)):
-                raise ValueError("vel must be scalar for mode=analytical")
-            ts, tr, ds, dr, gs, gr = T.analytic(z, x, srcs, recs, vel, y, dims, dsamp, dev)
+                # the reference solves with scikit-fmm (K:609-638); here the
+                # device fast-sweeping solver (kx_eikonal)
+                ts, tr, ds, dr, gs, gr = T.eikonal_tables(srcs, recs, vel, origin, dsamp, dims,
+                                                          dev, self.dynamic)
+            else:
+                if not isinstance(vel, (float, int)):
+                    raise ValueError("vel must be scalar for mode=analytical")
+                ts, tr, ds, dr, gs, gr = T.analytic(z, x, srcs, recs, vel, y, dims, dsamp, dev)
             self._set("trav_srcs", ts)
             self._set("trav_recs", tr)
             if self.dynamic:
@@ -265,21 +268,22 @@ class Kirchhoff(LinearOperator):
 
     @staticmethod
     def _traveltime_table(z, x, srcs, recs, vel, y=None, mode="eikonal"):
-        """K:502-657 for mode="analytic" (numpy outputs, float64).  The
-        tables are computed with torch on the current HIP device when one
-        is visible and on the host otherwise."""
-        (ndims, _, dims, _, _, _, _, _, _, _, _, dsamp, _) = \
+        """K:502-657 (numpy outputs, float64).  mode="analytic" is computed
+        with torch on the current HIP device when one is visible and on the
+        host otherwise; mode="eikonal" runs the device eikonal solver."""
+        (ndims, _, dims, _, _, _, _, _, _, _, _, dsamp, origin) = \
             Kirchhoff._identify_geometry(z, x, srcs, recs, y=y)
         if mode == "analytic":
             if not isinstance(vel, (float, int)):
                 raise ValueError("vel must be scalar for mode=analytical")
+            dev = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
+            out = T.analytic(z, x, srcs, recs, vel, y, dims, dsamp, dev)
         elif mode == "eikonal":
-            raise NotImplementedError(
-                "mode='eikonal' needs an eikonal solver, which is not provided")
+            _lib.require_device()
+            dev = torch.device("cuda", torch.cuda.current_device())
+            out = T.eikonal_tables(srcs, recs, vel, origin, dsamp, dims, dev, True)
         else:
             raise NotImplementedError("method must be analytic or eikonal")
-        dev = torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu")
-        out = T.analytic(z, x, srcs, recs, vel, y, dims, dsamp, dev)
         ts, tr, ds, dr = (o.cpu().numpy() for o in out[:4])
         gs = [g.cpu().numpy() for g in out[4]]
         gr = [g.cpu().numpy() for g in out[5]]

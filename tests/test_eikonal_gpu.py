@@ -1,0 +1,163 @@
+"""Eikonal traveltime tables on the device (kx_eikonal) and Kirchhoff's
+mode="eikonal" (pylops/waveeqprocessing/kirchhoff.py:609-638).
+
+The reference solves with scikit-fmm, which is absent here; the device
+solver is checked against the oracle's restatement of the same sweeps
+(oracle.eikonal_fsm, itself pinned by closed forms in test_oracle.py) and,
+as the reference's own tests do (pytests/test_kirchhoff.py:160-212, the
+eikonal cases of test_kirchhoff2d/3d), against the analytic tables to two
+decimals and by the dot test."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _device_tables(slow, src, h, unit=False):
+    """kx_eikonal through the C-ABI for slow[ny, nx, nz] and source cells
+    (ix, iy, iz); returns (T[n, ny, nx, nz], rounds)."""
+    import ctypes
+    from pylops_amd import _lib
+    ny, nx, nz = slow.shape
+    ndims = 3 if ny > 1 else 2
+    n = len(src)
+    s = torch.as_tensor(np.ascontiguousarray(slow, dtype=np.float64).ravel(), device=DEV)
+    sd = torch.as_tensor(np.asarray(src, dtype=np.int32).reshape(-1, 3), device=DEV)
+    out = torch.empty((n, ny * nx * nz), dtype=torch.float64, device=DEV)
+    flag = torch.zeros(1, dtype=torch.int32, device=DEV)
+    it = ctypes.c_int32(0)
+    L = _lib.load()
+    rc = L.kx_eikonal(ndims, nx, ny, nz, h[0], h[1], h[2], _lib.ptr(s), n, _lib.ptr(sd),
+                      _lib.ptr(out), 20, _lib.ptr(flag), ctypes.byref(it),
+                      _lib.stream_ptr(torch.device(DEV)))
+    _lib.check(rc, "kx_eikonal")
+    return out.cpu().numpy().reshape(n, ny, nx, nz), it.value
+
+
+@pytest.mark.parametrize("case", ["2d_const", "2d_vz", "3d_const", "3d_vz", "2d_lens"])
+def test_device_matches_oracle(case):
+    """Same sweep order, same arithmetic: device and oracle agree to
+    rounding, and need the same number of rounds."""
+    rng = np.random.default_rng(11)
+    if case.startswith("2d"):
+        ny, nx, nz, h = 1, 37, 29, (7.5, 1.0, 5.0)
+    else:
+        ny, nx, nz, h = 9, 23, 17, (10.0, 12.5, 8.0)
+    z = np.arange(nz) * h[2]
+    if case.endswith("const"):
+        v = np.full((ny, nx, nz), 1800.0)
+    elif case.endswith("vz"):
+        v = np.broadcast_to(1500.0 + 2.0 * z, (ny, nx, nz)).copy()
+    else:
+        X, Z = np.meshgrid(np.arange(nx) * h[0], z, indexing="ij")
+        lens = 1.0 - 0.3 * np.exp(-((X - 130) ** 2 + (Z - 70) ** 2) / 1500.0)
+        v = (2000.0 * lens)[None]
+    src = [(int(rng.integers(nx)), int(rng.integers(ny)), int(rng.integers(nz)))
+           for _ in range(5)]
+    src.append((0, 0, 0))
+    src.append((nx - 1, ny - 1, nz - 1))
+    got, it_dev = _device_tables(1.0 / v, src, h)
+    want, it_ora = O.eikonal_fsm(1.0 / v, src, h)
+    assert it_dev == it_ora, (it_dev, it_ora)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+
+
+def test_traveltime_table_matches_analytic():
+    """pytests/test_kirchhoff.py:160-212: eikonal and analytic tables of a
+    constant medium agree to two decimals, 2D and 3D."""
+    from pylops_amd.waveeqprocessing import Kirchhoff
+    ts_a, tr_a, _, _, _, _ = Kirchhoff._traveltime_table(G.z, G.x, G.s2d, G.r2d, G.v0,
+                                                         mode="analytic")
+    ts_e, tr_e, ds_e, dr_e, gs_e, gr_e = Kirchhoff._traveltime_table(
+        G.z, G.x, G.s2d, G.r2d, G.v0 * np.ones((G.PAR["nx"], G.PAR["nz"])), mode="eikonal")
+    np.testing.assert_array_almost_equal(ts_a, ts_e, decimal=2)
+    np.testing.assert_array_almost_equal(tr_a, tr_e, decimal=2)
+    # unit slowness gives the distance, and times are distance / v
+    np.testing.assert_allclose(ds_e / G.v0, ts_e, rtol=1e-12, atol=1e-15)
+    assert gs_e[0].shape == (G.PAR["nx"], G.PAR["nz"], G.PAR["nsx"])
+    ts_a, tr_a, _, _, _, _ = Kirchhoff._traveltime_table(G.z, G.x, G.s3d_par, G.r3d_par, G.v0,
+                                                         y=G.y, mode="analytic")
+    ts_e, tr_e, _, _, _, _ = Kirchhoff._traveltime_table(
+        G.z, G.x, G.s3d_par, G.r3d_par,
+        G.v0 * np.ones((G.PAR["ny"], G.PAR["nx"], G.PAR["nz"])), y=G.y, mode="eikonal")
+    np.testing.assert_array_almost_equal(ts_a, ts_e, decimal=2)
+    np.testing.assert_array_almost_equal(tr_a, tr_e, decimal=2)
+
+
+def test_table_gradients_follow_numpy():
+    """K:641-647: the gradients are np.gradient of the eikonal tables."""
+    from pylops_amd.waveeqprocessing import Kirchhoff
+    vel = np.broadcast_to(1500.0 + 20.0 * G.z, (G.PAR["nx"], G.PAR["nz"])).copy()
+    ts, tr, _, _, gs, gr = Kirchhoff._traveltime_table(G.z, G.x, G.s2d, G.r2d, vel,
+                                                       mode="eikonal")
+    ref = np.gradient(ts.reshape(G.PAR["nx"], G.PAR["nz"], -1), G.PAR["dx"], G.PAR["dz"],
+                      axis=(0, 1))
+    for a, b in zip(gs, ref):
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_kirchhoff2d_eikonal_dottest(dynamic):
+    """The eikonal cases of pytests/test_kirchhoff.py:215-262 (par2, par2d)."""
+    from pylops_amd.utils import dottest
+    from pylops_amd.waveeqprocessing import Kirchhoff
+    vel = G.v0 * np.ones((G.PAR["nx"], G.PAR["nz"]))
+    Dop = Kirchhoff(G.z, G.x, G.t, G.s2d, G.r2d, vel, G.wav, G.wavc, mode="eikonal",
+                    dynamic=dynamic)
+    assert dottest(Dop, G.PAR["nsx"] * G.PAR["nrx"] * G.PAR["nt"], G.PAR["nz"] * G.PAR["nx"])
+
+
+def test_kirchhoff3d_eikonal_dottest():
+    """The eikonal case of pytests/test_kirchhoff.py:265-318 (par2)."""
+    from pylops_amd.utils import dottest
+    from pylops_amd.waveeqprocessing import Kirchhoff
+    vel = G.v0 * np.ones((G.PAR["ny"], G.PAR["nx"], G.PAR["nz"]))
+    Dop = Kirchhoff(G.z, G.x, G.t, G.s3d_par, G.r3d_par, vel, G.wav, G.wavc, y=G.y,
+                    mode="eikonal")
+    assert dottest(Dop, G.PAR["nsx"] * G.PAR["nrx"] * G.PAR["nsy"] * G.PAR["nry"] * G.PAR["nt"],
+                   G.PAR["nz"] * G.PAR["nx"] * G.PAR["ny"])
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_kirchhoff_eikonal_uses_device_tables(dynamic):
+    """The operator's tables are the solver's, and its products match the
+    oracle operator built on the same tables (byot).  byot takes its angles
+    from unit-spacing gradients (K:351-360), which give the same angles as
+    the eikonal mode's only on a grid with dx == dz: the dynamic case uses
+    one."""
+    from pylops_amd.waveeqprocessing import Kirchhoff
+    x = G.x * 2.0 if dynamic else G.x
+    s = np.vstack((np.linspace(x.min(), x.max(), G.PAR["nsx"]), G.s2d[1]))
+    r = np.vstack((np.linspace(x.min(), x.max(), G.PAR["nrx"]), G.r2d[1]))
+    vel = np.broadcast_to(1500.0 + 20.0 * G.z, (G.PAR["nx"], G.PAR["nz"])).copy()
+    Dop = Kirchhoff(G.z, x, G.t, s, r, vel, G.wav, G.wavc, mode="eikonal", dynamic=dynamic)
+    ts, tr, ds, dr, _, _ = Kirchhoff._traveltime_table(G.z, x, s, r, vel, mode="eikonal")
+    np.testing.assert_allclose(np.asarray(Dop.trav_srcs), ts, rtol=1e-12)
+    np.testing.assert_allclose(np.asarray(Dop.trav_recs), tr, rtol=1e-12)
+    amp = (np.asarray(Dop.amp_srcs), np.asarray(Dop.amp_recs)) if dynamic else None
+    ref = O.KirchhoffOracle(G.z, x, G.t, s, r, vel, G.wav, G.wavc, mode="byot",
+                            dynamic=dynamic, trav=(ts, tr), amp=amp)
+    m = np.random.default_rng(3).standard_normal(G.PAR["nx"] * G.PAR["nz"])
+    d = np.asarray(Dop.matvec(m))
+    np.testing.assert_allclose(d, ref.matvec(m), rtol=1e-10, atol=1e-12 * np.abs(d).max())
+    dd = np.random.default_rng(4).standard_normal(d.size)
+    a = np.asarray(Dop.rmatvec(dd))
+    np.testing.assert_allclose(a, ref.rmatvec(dd), rtol=1e-10, atol=1e-12 * np.abs(a).max())
+
+
+def test_source_outside_grid_raises():
+    from pylops_amd.waveeqprocessing import Kirchhoff
+    s = G.s2d.copy()
+    s[0, 0] = -5.0
+    with pytest.raises(ValueError):
+        Kirchhoff._traveltime_table(G.z, G.x, s, G.r2d,
+                                    G.v0 * np.ones((G.PAR["nx"], G.PAR["nz"])), mode="eikonal")
+    # the C-ABI checks the cells itself (KX_ESHAPE)
+    with pytest.raises(ValueError, match="outside"):
+        _device_tables(np.ones((1, 5, 4)), [(5, 0, 0)], (1.0, 1.0, 1.0))

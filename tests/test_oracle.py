@@ -234,3 +234,64 @@ def test_fista_ista_recovery_as_reference_tests(shape):
             xinv, _, _ = O.shrinkage(O.DenseOp(A), y, alpha, niter=2000, eps=0.5, tol=0,
                                      threshkind=kind, fast=fast)
             np.testing.assert_array_almost_equal(x, xinv, decimal=1)
+
+
+# ---------------------------------------------------------------------------
+# eikonal traveltimes (oracle.eikonal_fsm); the reference's scikit-fmm is
+# absent, so the restatement is pinned by closed forms
+# ---------------------------------------------------------------------------
+def test_eikonal_one_dimensional_is_exact():
+    """On a one-cell-deep grid only the x update is active: T = s (|k| h -
+    h/2), the contour half a cell from the source centre."""
+    T, _ = O.eikonal_fsm(np.full((1, 30, 1), 1 / 500.0), [(10, 0, 0)], (2.0, 1.0, 3.0))
+    k = np.arange(30) - 10
+    np.testing.assert_allclose(T[0, 0, :, 0], np.maximum(np.abs(k) * 2.0 - 1.0, 0) / 500.0,
+                               rtol=0, atol=1e-15)
+
+
+def test_eikonal_constant_velocity_matches_analytic():
+    """pytests/test_kirchhoff.py:163-212 restated: eikonal and analytic
+    traveltimes agree to two decimals (seconds) on the reference's grids,
+    2D and 3D, with unequal spacings."""
+    from tests import geometry as G
+    for dim3 in (False, True):
+        s = G.s3d if dim3 else G.s2d
+        if dim3:
+            ny, nx, nz = len(G.y), len(G.x), len(G.z)
+            h = (G.x[1] - G.x[0], G.y[1] - G.y[0], G.z[1] - G.z[0])
+            src = [(int(round(p[1] / h[0])), int(round(p[0] / h[1])), int(round(p[2] / h[2])))
+                   for p in s.T]
+        else:
+            ny, nx, nz = 1, len(G.x), len(G.z)
+            h = (G.x[1] - G.x[0], 1.0, G.z[1] - G.z[0])
+            src = [(int(round(p[0] / h[0])), 0, int(round(p[1] / h[2]))) for p in s.T]
+        T, _ = O.eikonal_fsm(np.full((ny, nx, nz), 1 / G.v0), src, h)
+        ana = O.analytic_tables(G.z, G.x, s, s, G.v0, G.y if dim3 else None)[0]
+        np.testing.assert_array_almost_equal(T.reshape(len(src), -1).T, ana, decimal=2)
+
+
+def test_eikonal_symmetry():
+    """Constant velocity, source at the centre of a symmetric grid: every
+    sweep order is used, so the converged field is mirror symmetric."""
+    T, _ = O.eikonal_fsm(np.full((1, 21, 17), 1 / 2000.0), [(10, 0, 8)], (5.0, 1.0, 7.0))
+    np.testing.assert_allclose(T[0, 0], T[0, 0, ::-1, :], rtol=1e-14)
+    np.testing.assert_allclose(T[0, 0], T[0, 0, :, ::-1], rtol=1e-14)
+
+
+def test_eikonal_linear_gradient_converges_first_order():
+    """v(z) = v0 + g z has the closed-form first arrival
+    t = acosh(1 + g^2 r^2 / (2 v(z_s) v(z))) / g; away from the source the
+    error halves with the spacing (first-order scheme)."""
+    def err(h):
+        n = int(400.0 / h) + 1
+        x = np.arange(n) * h
+        X, Z = np.meshgrid(x, x, indexing="ij")
+        v = 1500.0 + Z
+        ix, iz = n // 2, n // 4
+        T, _ = O.eikonal_fsm((1 / v)[None], [(ix, 0, iz)], (h, 1.0, h))
+        r2 = (X - x[ix]) ** 2 + (Z - x[iz]) ** 2
+        ex = np.arccosh(1 + r2 / (2 * (1500.0 + x[iz]) * v))
+        far = np.sqrt(r2) > 100.0
+        return np.max(np.abs(T[0, 0] - ex)[far] / ex[far])
+    e1, e2, e3 = err(10.0), err(5.0), err(2.5)
+    assert e1 < 0.06 and 1.7 < e1 / e2 < 2.3 and 1.7 < e2 / e3 < 2.3, (e1, e2, e3)
