@@ -312,19 +312,21 @@ int kx_cgls_scalar_batch(double *sc, int32_t nseg, double *cost, double *cost1, 
 
 /* ---- traveltime tables by the eikonal equation ----
  * Replaces the scikit-fmm calls of Kirchhoff._traveltime_table, mode
- * "eikonal" (K:609-638): T[s][ni] (reference point order ii) is the
- * first-arrival time from the zero contour of phi = -1 at source cell
- * src[s] = (ix, iy, iz) and +1 elsewhere, for the slowness field slow[ni]
+ * "eikonal" (K:609-638): T[ii * n + s] (the reference's (ni, n) table,
+ * point order ii) is the first-arrival time from the zero contour of
+ * phi = -1 at source cell src[s] = (ix, iy, iz) and +1 elsewhere
+ * (1 <= n <= 2^22), for the slowness field slow[ni]
  * (unit slowness gives the distance, skfmm.distance).  Fast sweeping with
- * first-order Godunov updates until no value changes (at most max_iter
- * rounds of all sweep directions); *iters (host) receives the rounds run.
- * 2D: ndims = 2 and ny is ignored (iy = 0).  src (device) is copied back
- * and every cell checked against the grid before any launch (KX_ESHAPE).
- * flag: one device int32 of scratch.  Synchronises the stream (not
- * capturable). */
+ * Godunov updates: first order until no value changes, then, for order 2
+ * (scikit-fmm's default), second-order one-sided differences until no value
+ * changes; each phase runs at most max_iter rounds of all sweep directions,
+ * and *iters (host) receives the rounds run.  2D: ndims = 2 and ny is
+ * ignored (iy = 0).  src (device) is copied back and every cell checked
+ * against the grid before any launch (KX_ESHAPE).  flag: one device int32
+ * of scratch.  Synchronises the stream (not capturable). */
 int kx_eikonal(int32_t ndims, int32_t nx, int32_t ny, int32_t nz, double hx, double hy,
                double hz, const double *slow, int32_t n, const int32_t *src, double *T,
-               int32_t max_iter, int32_t *flag, int32_t *iters, void *stream);
+               int32_t max_iter, int32_t order, int32_t *flag, int32_t *iters, void *stream);
 
 #ifdef __cplusplus
 }

@@ -601,54 +601,84 @@ _EIK_BIG = 1e30
 _EIK_R = 2
 
 
-def _godunov(ax, ay, az, hx, hy, hz, sl, three):
-    """Largest t with sum_k ((t - a_k)/h_k)^2 = s^2 over the axes below t
-    (first-order Godunov upwind), vectorised; same operation order as the
-    device kernel."""
-    a0, a1, a2 = ax.copy(), az.copy(), ay.copy()
-    h0 = np.full_like(a0, hx)
-    h1 = np.full_like(a0, hz)
-    h2 = np.full_like(a0, hy)
+def _solve_upwind(a, b, g, sl, three):
+    """Largest t with sum_k ((t - b_k)/g_k)^2 = s^2 over the axes whose
+    upwind value a_k lies below t, taken in increasing a_k: the Godunov
+    update with one-sided differences (t - b_k)/g_k per axis (first order:
+    b = a, g = h; second order: b = (4a - a2)/3, g = 2h/3).  Vectorised,
+    same operation order as the device kernel.  Returns (t, ok) with ok
+    False where a quadratic had no real root."""
+    a0, a1, a2 = a
+    b0, b1, b2 = b
+    g0, g1, g2 = g
 
-    def swap(u, v, hu, hv):
-        m = v < u
-        return (np.where(m, v, u), np.where(m, u, v), np.where(m, hv, hu), np.where(m, hu, hv))
+    def swap(u, v):
+        m = v[0] < u[0]
+        return (tuple(np.where(m, y, x) for x, y in zip(u, v)),
+                tuple(np.where(m, x, y) for x, y in zip(u, v)))
 
-    a0, a1, h0, h1 = swap(a0, a1, h0, h1)
+    (a0, b0, g0), (a1, b1, g1) = swap((a0, b0, g0), (a1, b1, g1))
     if three:
-        a1, a2, h1, h2 = swap(a1, a2, h1, h2)
-        a0, a1, h0, h1 = swap(a0, a1, h0, h1)
-    t1 = a0 + sl * h0
-    w0 = 1.0 / (h0 * h0)
-    w1 = 1.0 / (h1 * h1)
+        (a1, b1, g1), (a2, b2, g2) = swap((a1, b1, g1), (a2, b2, g2))
+        (a0, b0, g0), (a1, b1, g1) = swap((a0, b0, g0), (a1, b1, g1))
+    t1 = b0 + sl * g0
+    w0 = 1.0 / (g0 * g0)
+    w1 = 1.0 / (g1 * g1)
     A = w0 + w1
-    B = a0 * w0 + a1 * w1
-    C = a0 * a0 * w0 + a1 * a1 * w1 - sl * sl
-    t2 = (B + np.sqrt(np.maximum(B * B - A * C, 0.0))) / A
+    B = b0 * w0 + b1 * w1
+    C = b0 * b0 * w0 + b1 * b1 * w1 - sl * sl
+    D = B * B - A * C
+    t2 = (B + np.sqrt(np.maximum(D, 0.0))) / A
     m2 = t1 > a1
     t = np.where(m2, t2, t1)
+    ok = ~m2 | (D >= 0.0)
     if three:
-        w2 = 1.0 / (h2 * h2)
+        w2 = 1.0 / (g2 * g2)
         A3 = A + w2
-        B3 = B + a2 * w2
-        C3 = C + a2 * a2 * w2
-        t3 = (B3 + np.sqrt(np.maximum(B3 * B3 - A3 * C3, 0.0))) / A3
-        t = np.where(m2 & (t2 > a2), t3, t)
-    return t
+        B3 = B + b2 * w2
+        C3 = C + b2 * b2 * w2
+        D3 = B3 * B3 - A3 * C3
+        t3 = (B3 + np.sqrt(np.maximum(D3, 0.0))) / A3
+        m3 = m2 & (t2 > a2)
+        t = np.where(m3, t3, t)
+        ok = ok & (~m3 | (D3 >= 0.0))
+    return t, ok
 
 
-def eikonal_fsm(slow, src, h, max_iter=20):
+def _axis_terms(l1, r1, l2, r2, h, order):
+    """Upwind value a, difference base b and step g of one axis from the
+    neighbours one (l1, r1) and two (l2, r2) cells away on each side
+    (_EIK_BIG where absent): the smaller side is upwind; second order uses
+    (3t - 4a + a2)/(2h) when the second cell on that side is known and not
+    above the first."""
+    left = l1 <= r1
+    a = np.where(left, l1, r1)
+    h = np.broadcast_to(np.asarray(h, dtype=np.float64), a.shape)
+    if order == 1:
+        return a, a, h
+    a2 = np.where(left, l2, r2)
+    two = (a < _EIK_BIG) & (a2 < _EIK_BIG) & (a2 <= a)
+    return a, np.where(two, (4.0 * a - a2) / 3.0, a), np.where(two, (2.0 / 3.0) * h, h)
+
+
+def eikonal_fsm(slow, src, h, max_iter=20, order=2):
     """First-arrival times T[n][ny, nx, nz] from the zero contour of phi = -1
     at each source cell src[s] = (ix, iy, iz) and +1 elsewhere, for the
-    slowness slow[ny, nx, nz] (2D: ny = 1, hy unused), by fast sweeping with
-    first-order Godunov updates.  Cells within two cells (per axis) of the
-    source hold the straight-ray time from the contour (a diamond through
-    the half-way points to the neighbouring cell centres) and stay fixed.
-    Returns (T, rounds)."""
+    slowness slow[ny, nx, nz] (2D: ny = 1, hy unused), by fast sweeping
+    (Zhao 2005) with Godunov upwind updates.  Cells within two cells (per
+    axis) of the source hold the straight-ray time from the contour (a
+    diamond through the half-way points to the neighbouring cell centres)
+    and stay fixed.  First-order sweeps run to convergence; with order 2,
+    second-order sweeps (one-sided differences over two cells where the
+    upwind values are monotone, first order where the quadratic has no root)
+    then lower the values further until they converge too.  Returns
+    (T, rounds), rounds counting both phases."""
     slow = np.asarray(slow, dtype=np.float64)
     ny, nx, nz = slow.shape
     hx, hy, hz = h
     three = ny > 1
+    if not three:
+        hy = 1.0
     src = np.asarray(src, dtype=np.int64).reshape(-1, 3)
     n = len(src)
     IY, IX, IZ = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
@@ -658,7 +688,7 @@ def eikonal_fsm(slow, src, h, max_iter=20):
         box = ((np.abs(IX - sx) <= _EIK_R) & (np.abs(IY - sy) <= _EIK_R)
                & (np.abs(IZ - sz) <= _EIK_R))
         dx = (IX - sx) * hx
-        dy = (IY - sy) * (hy if three else 1.0)
+        dy = (IY - sy) * hy
         dz = (IZ - sz) * hz
         rho = np.sqrt(dx * dx + dy * dy + dz * dz)
         S = 2.0 * (np.abs(dx) / hx + np.abs(dz) / hz + (np.abs(dy) / hy if three else 0.0))
@@ -669,42 +699,64 @@ def eikonal_fsm(slow, src, h, max_iter=20):
         frozen[k] = box
     ndiag = (nx - 1) + (ny - 1) + (nz - 1) + 1
     ndir = 8 if three else 4
+    BIG = _EIK_BIG
+
+    def nb(cy, cx, cz, axis, off):
+        """T at the neighbour off cells along axis, BIG outside the grid."""
+        if axis == 0:
+            j, nn = cx + off, nx
+            v = T[:, cy, np.clip(j, 0, nx - 1), cz]
+        elif axis == 1:
+            j, nn = cy + off, ny
+            v = T[:, np.clip(j, 0, ny - 1), cx, cz]
+        else:
+            j, nn = cz + off, nz
+            v = T[:, cy, cx, np.clip(j, 0, nz - 1)]
+        return np.where((j >= 0) & (j < nn), v, BIG)
+
     rounds = 0
-    for _ in range(max_iter):
-        rounds += 1
-        changed = False
-        for q in range(ndir):
-            fx = (q & 1) != 0
-            fz = (q & 2) != 0
-            fy = (q & 4) != 0
-            PX = np.where(fx, nx - 1 - IX, IX)
-            PY = np.where(fy, ny - 1 - IY, IY)
-            PZ = np.where(fz, nz - 1 - IZ, IZ)
-            diag = PX + PY + PZ
-            for d in range(ndiag):
-                cy, cx, cz = np.nonzero(diag == d)
-                if len(cx) == 0:
-                    continue
-                Tc = T[:, cy, cx, cz]
-                BIG = _EIK_BIG
-                ax = np.minimum(np.where(cx > 0, T[:, cy, np.maximum(cx - 1, 0), cz], BIG),
-                                np.where(cx < nx - 1, T[:, cy, np.minimum(cx + 1, nx - 1), cz], BIG))
-                az = np.minimum(np.where(cz > 0, T[:, cy, cx, np.maximum(cz - 1, 0)], BIG),
-                                np.where(cz < nz - 1, T[:, cy, cx, np.minimum(cz + 1, nz - 1)], BIG))
-                if three:
-                    ay = np.minimum(
-                        np.where(cy > 0, T[:, np.maximum(cy - 1, 0), cx, cz], BIG),
-                        np.where(cy < ny - 1, T[:, np.minimum(cy + 1, ny - 1), cx, cz], BIG))
-                else:
-                    ay = np.full_like(ax, BIG)
-                t = _godunov(ax, ay, az, hx, hy if three else 1.0, hz,
-                             np.broadcast_to(slow[cy, cx, cz], Tc.shape), three)
-                upd = (t < Tc) & ~frozen[:, cy, cx, cz]
-                if np.any(upd & (Tc - t > 1e-13 * t)):
-                    changed = True
-                T[:, cy, cx, cz] = np.where(upd, t, Tc)
-        if not changed:
-            break
+    for phase in ((1, 2) if order == 2 else (1,)):
+        for _ in range(max_iter):
+            rounds += 1
+            changed = False
+            for q in range(ndir):
+                fx = (q & 1) != 0
+                fz = (q & 2) != 0
+                fy = (q & 4) != 0
+                PX = np.where(fx, nx - 1 - IX, IX)
+                PY = np.where(fy, ny - 1 - IY, IY)
+                PZ = np.where(fz, nz - 1 - IZ, IZ)
+                diag = PX + PY + PZ
+                for d in range(ndiag):
+                    cy, cx, cz = np.nonzero(diag == d)
+                    if len(cx) == 0:
+                        continue
+                    Tc = T[:, cy, cx, cz]
+                    sl = np.broadcast_to(slow[cy, cx, cz], Tc.shape)
+                    terms = []
+                    for axis, hh in ((0, hx), (2, hz), (1, hy)):
+                        if axis == 1 and not three:
+                            big = np.full_like(Tc, BIG)
+                            terms.append((big, big, np.broadcast_to(hh, Tc.shape)))
+                            continue
+                        l2 = nb(cy, cx, cz, axis, -2) if phase == 2 else None
+                        r2 = nb(cy, cx, cz, axis, 2) if phase == 2 else None
+                        terms.append(_axis_terms(nb(cy, cx, cz, axis, -1),
+                                                 nb(cy, cx, cz, axis, 1), l2, r2, hh, phase))
+                    a = tuple(tt[0] for tt in terms)
+                    b = tuple(tt[1] for tt in terms)
+                    g = tuple(tt[2] for tt in terms)
+                    t, ok = _solve_upwind(a, b, g, sl, three)
+                    if phase == 2:
+                        hs = tuple(np.broadcast_to(hh, Tc.shape) for hh in (hx, hz, hy))
+                        t1, _ = _solve_upwind(a, a, hs, sl, three)
+                        t = np.where(ok, t, t1)
+                    upd = (t < Tc) & ~frozen[:, cy, cx, cz]
+                    if np.any(upd & (Tc - t > 1e-13 * t)):
+                        changed = True
+                    T[:, cy, cx, cz] = np.where(upd, t, Tc)
+            if not changed:
+                break
     return T, rounds
 
 

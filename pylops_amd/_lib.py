@@ -206,7 +206,7 @@ def _declare(L):
         "kx_seg_vec": ([I32, PS, I32, P, P, P, I32, I32, D, I32, I32, P, P], ctypes.c_int),
         "kx_seg_dot": ([PS, I32, P, P, P, I32, I32, I32, P, P], ctypes.c_int),
         "kx_cgls_scalar_batch": ([P, I32, P, P, I32, I32, P], ctypes.c_int),
-        "kx_eikonal": ([I32, I32, I32, I32, D, D, D, P, I32, P, P, I32, P,
+        "kx_eikonal": ([I32, I32, I32, I32, D, D, D, P, I32, P, P, I32, I32, P,
                         ctypes.POINTER(I32), P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

@@ -67,12 +67,12 @@ def analytic(z, x, srcs, recs, vel, y, dims, dsamp, device):
     return ts, tr, ds, dr, gradient(ts, dims, dsamp), gradient(tr, dims, dsamp)
 
 
-def eikonal(pts, vel, origin, dsamp, dims, device, unit=False, max_iter=20):
+def eikonal(pts, vel, origin, dsamp, dims, device, unit=False, max_iter=20, order=2):
     """Traveltime (or, with ``unit``, distance) tables of shape (ni, n) from
     the grid cells nearest to the points ``pts`` ((x, z) or (y, x, z) rows,
     rounded as K:617-633 rounds them), by the device eikonal solver
-    (``kx_eikonal``).  ``vel`` is a scalar or an array of the grid's
-    shape."""
+    (``kx_eikonal``; ``order`` 2 is scikit-fmm's default).  ``vel`` is a
+    scalar or an array of the grid's shape."""
     import ctypes
     ndims = len(dims)
     pts = np.asarray(pts, dtype=np.float64)
@@ -99,15 +99,15 @@ def eikonal(pts, vel, origin, dsamp, dims, device, unit=False, max_iter=20):
         slow = 1.0 / v
     n = len(src)
     srcd = torch.as_tensor(np.ascontiguousarray(src, dtype=np.int32), device=device)
-    out = torch.empty((n, ni), dtype=torch.float64, device=device)
+    out = torch.empty((ni, n), dtype=torch.float64, device=device)
     flag = torch.zeros(1, dtype=torch.int32, device=device)
     iters = ctypes.c_int32(0)
     L = _lib.load()
     rc = L.kx_eikonal(ndims, nx, ny, nz, hx, hy, hz, _lib.ptr(slow.contiguous()), n,
-                      _lib.ptr(srcd), _lib.ptr(out), max_iter, _lib.ptr(flag),
+                      _lib.ptr(srcd), _lib.ptr(out), max_iter, order, _lib.ptr(flag),
                       ctypes.byref(iters), _lib.stream_ptr(device))
     _lib.check(rc, "kx_eikonal")
-    return out.T
+    return out
 
 
 def eikonal_tables(srcs, recs, vel, origin, dsamp, dims, device, distances):

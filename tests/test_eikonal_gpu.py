@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda"
 
 
-def _device_tables(slow, src, h, unit=False):
+def _device_tables(slow, src, h, order=2):
     """kx_eikonal through the C-ABI for slow[ny, nx, nz] and source cells
     (ix, iy, iz); returns (T[n, ny, nx, nz], rounds)."""
     import ctypes
@@ -29,19 +29,20 @@ def _device_tables(slow, src, h, unit=False):
     n = len(src)
     s = torch.as_tensor(np.ascontiguousarray(slow, dtype=np.float64).ravel(), device=DEV)
     sd = torch.as_tensor(np.asarray(src, dtype=np.int32).reshape(-1, 3), device=DEV)
-    out = torch.empty((n, ny * nx * nz), dtype=torch.float64, device=DEV)
+    out = torch.empty((ny * nx * nz, n), dtype=torch.float64, device=DEV)
     flag = torch.zeros(1, dtype=torch.int32, device=DEV)
     it = ctypes.c_int32(0)
     L = _lib.load()
     rc = L.kx_eikonal(ndims, nx, ny, nz, h[0], h[1], h[2], _lib.ptr(s), n, _lib.ptr(sd),
-                      _lib.ptr(out), 20, _lib.ptr(flag), ctypes.byref(it),
+                      _lib.ptr(out), 20, order, _lib.ptr(flag), ctypes.byref(it),
                       _lib.stream_ptr(torch.device(DEV)))
     _lib.check(rc, "kx_eikonal")
-    return out.cpu().numpy().reshape(n, ny, nx, nz), it.value
+    return out.cpu().numpy().T.reshape(n, ny, nx, nz), it.value
 
 
+@pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("case", ["2d_const", "2d_vz", "3d_const", "3d_vz", "2d_lens"])
-def test_device_matches_oracle(case):
+def test_device_matches_oracle(case, order):
     """Same sweep order, same arithmetic: device and oracle agree to
     rounding, and need the same number of rounds."""
     rng = np.random.default_rng(11)
@@ -62,8 +63,8 @@ def test_device_matches_oracle(case):
            for _ in range(5)]
     src.append((0, 0, 0))
     src.append((nx - 1, ny - 1, nz - 1))
-    got, it_dev = _device_tables(1.0 / v, src, h)
-    want, it_ora = O.eikonal_fsm(1.0 / v, src, h)
+    got, it_dev = _device_tables(1.0 / v, src, h, order)
+    want, it_ora = O.eikonal_fsm(1.0 / v, src, h, order=order)
     assert it_dev == it_ora, (it_dev, it_ora)
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
 
