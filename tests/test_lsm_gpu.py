@@ -1,6 +1,6 @@
 """LSM application (pytests/test_lsm.py:60-113 restated on the device
-solvers).  The eikonal cases of the reference need scikit-fmm, which the
-reference test also skips without it; mode="eikonal" raises here."""
+solvers), analytic and eikonal tables (the eikonal ones from the device
+eikonal solver; the reference needs scikit-fmm for them)."""
 import numpy as np
 import pytest
 from numpy.testing import assert_array_almost_equal
@@ -29,16 +29,19 @@ def test_unknown_mode():
         LSM(z, x, t, s2d, r2d, v0, wav, wavc, kind="twowayac")
 
 
+@pytest.mark.parametrize("mode", ["analytic", "eikonal"])
 @pytest.mark.parametrize("dynamic", [False, True])
-def test_lsm2d(dynamic):
-    """Dot-test and inversion (test_lsm.py:74-113): data fit to 2 decimals,
-    reflectivity to 1, with the scipy-lsqr-style default solver."""
+def test_lsm2d(dynamic, mode):
+    """Dot-test and inversion (test_lsm.py:74-113, par1/par2/par1d/par2d):
+    data fit to 2 decimals, reflectivity to 1, with the scipy-lsqr-style
+    default solver."""
     from pylops_amd.waveeqprocessing import LSM
+    vel = v0 * np.ones((PAR["nx"], PAR["nz"]))
     refl = np.zeros((PAR["nx"], PAR["nz"]))
     refl[:, PAR["nz"] // 2] = 1
     refl[:, 3 * PAR["nz"] // 4] = 1
-    lsm = LSM(z, x, t, s2d, r2d, v0, wav, wavc, mode="analytic", dynamic=dynamic,
-              dottest=True)
+    lsm = LSM(z, x, t, s2d, r2d, vel if mode == "eikonal" else v0, wav, wavc, mode=mode,
+              dynamic=dynamic, dottest=True)
     d = lsm.Demop * refl.ravel()
     d = d.reshape(PAR["nsx"], PAR["nrx"], PAR["nt"])
     minv = lsm.solve(d.ravel(), **dict(iter_lim=100, show=True))
