@@ -26,7 +26,7 @@ Workloads (synthetic inputs, fixed seeds):
            also 20 CGLS iterations per problem, batched and one by one.
   autograd the 2d operator as a torch autograd Function on 16 models: loss
            0.5|Km-d|^2 + 1e-3 TV(m), 5% noise, 50 Adam steps (a step is
-           one batched forward + backward + Adam, --adam-steps), and a
+           the forward of all 16 models + backward + Adam, --adam-steps), and a
            central-difference check of the gradient along 32 directions.
 
 Output: one JSON line.  value = 2 * ni * ns * nr / (t_fwd + t_adj) / 1e9
@@ -600,8 +600,9 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
     Gaussian noise (s = the clean data's standard deviation), 50 Adam steps
     from zero.  The gradient of the data term (the adjoint kernel) is
     checked against central differences along 32 random unit directions
-    with step 1e-3 |m|.  One step = batched forward + batched adjoint of all
-    16 models (one persistent launch each) + TV and Adam."""
+    with step 1e-3 |m|.  One step = forward of all 16 models + adjoint of
+    all 16 residuals (the operator's matmat / rmatmat: one launch of each
+    kernel per model) + TV and Adam."""
     import torch
     import pylops_amd
     from pylops_amd import TorchOperator
@@ -696,14 +697,14 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
                    "forward_ms": round(t_fwd * 1e3, 3),
                    "loss_backward_ms": round(t_bwd * 1e3, 3),
                    "s_total": round(wall, 3),
-                   "launch_shapes": {"rblock": Op._mm[1].rblock, "sorder": Op._mm[1].sorder,
-                                     "gsplit": Op._mm[1].gsplit},
+                   "launch_shapes": {"rblock": Op.plan.rblock, "sorder": Op.plan.sorder,
+                                     "gsplit": Op.plan.gsplit},
                    "loss_first": losses[0], "loss_last": losses[-1],
                    "model_rel_err_last": err_true,
                    "fd_check": {"step": "1e-3 |m|", "directions": ndir, "term": "data",
                                 "median_rel_err": float(np.median(errs)),
                                 "max_rel_err": float(np.max(errs))}},
-        "roofline": {"bound": "hbm", "kernel": "batched forward (16 models)",
+        "roofline": {"bound": "hbm", "kernel": "forward of 16 models",
                      "achieved": round(bytes_alg / t_fwd / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(bytes_alg / t_fwd / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "model_bytes": int(bytes_alg)},

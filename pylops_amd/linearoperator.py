@@ -49,6 +49,10 @@ class LinearOperator:
         self.explicit = False if explicit is None else explicit
         self.forceflat = forceflat
         self.name = name
+        self.reset_count()
+
+    def reset_count(self):
+        """Sets the four product counters back to 0 (LO:1230-1239)."""
         self.matvec_count = 0
         self.rmatvec_count = 0
         self.matmat_count = 0
