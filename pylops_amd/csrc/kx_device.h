@@ -245,11 +245,88 @@ __device__ __forceinline__ int round_half_up(float x)
 }
 __device__ __forceinline__ int round_half_up(double x) { return (int)floor(x + 0.5); }
 
+// (int)floor(x) in one conversion instruction (|x| < 2^31)
+__device__ __forceinline__ int floor_int(float x)
+{
+    int i;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
+
+// Accumulates one tile (64 points of a wave, RB receivers) into the LDS
+// traces.  The index and fraction of a float time come from one conversion
+// and one fract instruction: fract(T) is T - floor(T) rounded as the
+// subtraction rounds it, except that a result that would round to 1 (T in
+// (-2^-24, 0)) stays just below 1.  Dynamic weights are
+// (m ps) pr + (m qs) qr with the source-side products formed once per tile.
+template <typename TD, typename TT, bool DYN, bool NEG, bool FIX, int RB>
+__device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, RB> &cur, int iz, int e,
+                                                  const int (&lrow)[RB], const int (&izm)[RB],
+                                                  unsigned ntm1, double *accd,
+                                                  unsigned long long *acci)
+{
+    using TC = typename TimeType<TD>::type;
+    if (__ballot(cur.mv != TD(0)) == 0)
+        return;
+    const TC os = (TC)ld_off(&cur.os);
+    // the image value, scaled by 2^e for the fixed-point sum
+    TD mv = cur.mv;
+    if constexpr (FIX)
+        mv = ldexpf(mv, e);
+    TD mps = mv, mqs = TD(0);
+    if constexpr (DYN) {
+        mps = mv * cur.ps;
+        mqs = mv * cur.qs;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const TC T = os + (TC)ld_off(&cur.orr[r]);
+        TC f;
+        int k;
+        if constexpr (!NEG && sizeof(TC) == 4) {
+            f = __builtin_amdgcn_fractf(T);
+            k = floor_int(T) + cur.bs + cur.br[r];
+        } else {
+            const TC fl = floor(T);
+            f = T - fl;
+            k = (int)fl + cur.bs + cur.br[r];
+            if constexpr (NEG) {
+                // the reference truncates toward zero: a time in (-1, 0)
+                // lands on sample 0 with a negative weight
+                if (k == -1 && f > TC(0)) {
+                    k = 0;
+                    f -= TC(1);
+                }
+            }
+        }
+        TD w = mv;
+        bool on = lrow[r] >= 0 && (unsigned)k < ntm1;
+        if constexpr (DYN) {
+            w = fma(mqs, cur.qr[r], mps * cur.pr[r]);
+            on = on && iz >= izm[r];
+        }
+        if (on && w != TD(0)) {
+            const int it = lrow[r] + k;
+            const TD w1 = w * (TD)f;
+            if constexpr (FIX) {
+                // w (1 - f) in one rounding; fixed point by
+                // floor(x + 1/2) in one conversion
+                const TD w0 = fma(-w, (TD)f, w);
+                atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
+                atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
+            } else {
+                const TD w0 = w * (TD)(TC(1) - f);
+                atomicAdd(&accd[it], (double)w0);
+                atomicAdd(&accd[it + 1], (double)w1);
+            }
+        }
+    }
+}
+
 template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
 __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, int e, int s, int r0)
 {
-    using TC = typename TimeType<TD>::type;
     extern __shared__ __align__(16) unsigned char kx_smem[];
     double *accd = reinterpret_cast<double *>(kx_smem);
     unsigned long long *acci = reinterpret_cast<unsigned long long *>(kx_smem);
@@ -279,61 +356,21 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     }
     const unsigned ntm1 = (unsigned)(a.nt - 1);
 
+    // the next tile is loaded while this one is accumulated (a twice
+    // unrolled loop with two buffers and no copies measured slower: the
+    // second buffer costs registers and occupancy)
     SpreadTile<TD, TT, RB> cur, nxt;
     TileCursor c;
     c.start(a, wave);
     if (c.t < a.ntiles)
         spread_load<TD, TT, DYN, ROT, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, cur);
     while (c.t < a.ntiles) {
-        // prefetch the next tile while this one is accumulated
         TileCursor cn = c;
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             spread_load<TD, TT, DYN, ROT, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
-        if (__ballot(cur.mv != TD(0)) != 0) {
-            const int iz = c.tzi * a.tz + lz;
-            const TC os = (TC)ld_off(&cur.os);
-            // the image value, scaled by 2^e for the fixed-point sum
-            TD mv = cur.mv;
-            if constexpr (FIX)
-                mv = ldexpf(mv, e);
-#pragma unroll
-            for (int r = 0; r < RB; ++r) {
-                const TC T = os + (TC)ld_off(&cur.orr[r]);
-                const TC fl = floor(T);
-                TC f = T - fl;
-                int k = (int)fl + cur.bs + cur.br[r];
-                if constexpr (NEG) {
-                    // the reference truncates toward zero: a time in (-1, 0)
-                    // lands on sample 0 with a negative weight
-                    if (k == -1 && f > TC(0)) {
-                        k = 0;
-                        f -= TC(1);
-                    }
-                }
-                TD w = mv;
-                if constexpr (DYN) {
-                    w = w * (cur.ps * cur.pr[r] + cur.qs * cur.qr[r]);
-                    if (iz < izm[r])
-                        w = TD(0);
-                }
-                if (lrow[r] >= 0 && (unsigned)k < ntm1 && w != TD(0)) {
-                    const int it = lrow[r] + k;
-                    const TD w1 = w * (TD)f;
-                    if constexpr (FIX) {
-                        // w (1 - f) in one rounding; fixed point by
-                        // floor(x + 1/2) in one conversion
-                        const TD w0 = fma(-w, (TD)f, w);
-                        atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
-                        atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
-                    } else {
-                        const TD w0 = w * (TD)(TC(1) - f);
-                        atomicAdd(&accd[it], (double)w0);
-                        atomicAdd(&accd[it + 1], (double)w1);
-                    }
-                }
-            }
-        }
+        spread_accumulate<TD, TT, DYN, NEG, FIX, RB>(cur, c.tzi * a.tz + lz, e, lrow, izm, ntm1,
+                                                     accd, acci);
         cur = nxt;
         c = cn;
     }
