@@ -21,6 +21,9 @@
  *                  Kirchhoff._trav_kirch_rmatvec        K:704-725
  *   kx_conv     <- _Convolve1Dshort._matvec/_rmatvec    C1D:99-115
  *                  (scipy.signal.fftconvolve(mode="same"))
+ *   kx_conv_rows, kx_xcorr_rows
+ *               <- N-D filters of _Convolve1Dshort      C1D:57-115
+ *                  _Convolve1Dlong._matvec/_rmatvec     C1D:118-199
  *   kx_forward  <- Kirchhoff._matvec                    K:1048-1094
  *   kx_adjoint  <- Kirchhoff._rmatvec                   K:1096-1143
  *   kx_encode_table / kx_tile_table
@@ -137,6 +140,24 @@ int kx_gather(const kx_plan *p, const void *traces, void *m, void *stream);
  * filter h[nh]:  y[n] = sum_k h[k] x[n + (nh-1)/2 - k].  x != y. */
 int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const void *h,
             int32_t nh, int32_t dtype, void *stream);
+
+/* kx_conv with row strides: row r convolves x + r*xstride (nt samples) with
+ * the odd filter h + r*hstride (nh taps) into y + r*nt; a stride of 0
+ * shares one row.  Per-trace filters of Convolve1D (an N-D h,
+ * C1D:57-115) and the forward of a filter longer than the signal
+ * (_Convolve1Dlong._matvec, C1D:165-174: the long filter is the signal and
+ * the padded model the filter). */
+int kx_conv_rows(const void *x, int64_t xstride, void *y, int64_t ntr, int32_t nt,
+                 const void *h, int64_t hstride, int32_t nh, int32_t dtype, void *stream);
+
+/* Correlation at nlag lags: y[r*nlag + j] = sum_{u<la} a[r*astride + u] *
+ * b[r*bstride + lag0 + j + u], b zero outside [0, lb); summed in float64.
+ * The adjoint of a filter longer than the signal
+ * (_Convolve1Dlong._rmatvec, C1D:176-199: full convolution with the flipped
+ * filter, then the centre samples). */
+int kx_xcorr_rows(const void *a, int64_t astride, int32_t la, const void *b, int64_t bstride,
+                  int32_t lb, void *y, int64_t ntr, int32_t nlag, int32_t lag0, int32_t dtype,
+                  void *stream);
 
 /* Full operator: forward = spread then convolve with h; adjoint =
  * correlate (convolve with hstar) then gather.  work: kx_workspace_bytes. */

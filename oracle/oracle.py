@@ -13,7 +13,10 @@ Contents (reference file:line each part follows):
   -- K 283-301, 316-330, 343-393, 411-439
 * ``KirchhoffOracle`` -- the operator: kernels in ``liboracle.so`` (plain C,
   same loop nests as K 683-1001), orchestration K 1048-1143
-* ``conv_same`` -- Convolve1D short-filter path, convolve1d.py 57-115
+* ``conv_same`` / ``conv_same_rows`` -- Convolve1D short-filter path
+  (one filter, or one per trace), convolve1d.py 57-115
+* ``conv_long_forward`` / ``conv_long_adjoint`` -- the filter longer than
+  the signal, convolve1d.py 118-199
 * ``cgls`` / ``lsqr`` -- pylops/optimization/cls_basic.py 418-708, 835-1351
 * ``eikonal_fsm`` -- traveltime tables of the eikonal mode, K 609-638
   (the reference calls scikit-fmm, absent here: restated as fast sweeping
@@ -270,6 +273,48 @@ def conv_same(x2d, h_odd):
     lib().or_conv_same_f64(_ptr(x2d), _ptr(y), x2d.shape[0], x2d.shape[1],
                            _ptr(h), h.size)
     return y
+
+
+def conv_same_rows(x2d, h2d_odd):
+    """conv_same with one odd filter per row (an N-D filter of
+    _Convolve1Dshort, C1D:57-115, broadcast to the rows)."""
+    x2d = np.asarray(x2d, dtype=np.float64)
+    h2d = np.broadcast_to(np.asarray(h2d_odd, dtype=np.float64),
+                          (x2d.shape[0], np.shape(h2d_odd)[-1]))
+    return np.vstack([conv_same(x2d[r:r + 1], h2d[r]) for r in range(x2d.shape[0])])
+
+
+def _long_shift(n, offset):
+    """C1D:139-141: 2 (n//2 - offset), minus one for even n."""
+    sh = 2 * (n // 2 - int(offset))
+    return sh - 1 if n % 2 == 0 else sh
+
+
+def conv_long_forward(x, h, offset):
+    """_Convolve1Dlong._matvec (C1D:165-174) for one 1-D model x (n) and
+    filter h (nh > n): the model padded by the offset rule, then
+    fftconvolve(h, xp, mode="same"), i.e. the centre nh samples of the full
+    convolution."""
+    x = np.asarray(x, dtype=np.float64)
+    h = np.asarray(h, dtype=np.float64)
+    sh = _long_shift(x.size, offset)
+    xp = np.pad(x, (max(sh, 0), -min(sh, 0)))
+    full = np.convolve(h, xp)
+    s0 = (full.size - h.size) // 2
+    return full[s0:s0 + h.size]
+
+
+def conv_long_adjoint(d, h, offset, n):
+    """_Convolve1Dlong._rmatvec (C1D:176-199) for one 1-D data row d (nh):
+    the data padded the other way, fully convolved with the flipped filter,
+    and the n samples from len//2 - n//2 kept."""
+    d = np.asarray(d, dtype=np.float64)
+    h = np.asarray(h, dtype=np.float64)
+    sh = _long_shift(n, offset)
+    dp = np.pad(d, (-min(sh, 0), max(sh, 0)))
+    full = np.convolve(h[::-1], dp)
+    s0 = full.size // 2 - n // 2
+    return full[s0:s0 + n]
 
 
 # ---------------------------------------------------------------------------

@@ -104,11 +104,28 @@ __global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD 
 }
 
 template <typename TD>
-__global__ __launch_bounds__(kConvThreads) void conv_kernel(const TD *__restrict__ x, TD *__restrict__ y,
-                                                      int nt, const TD *__restrict__ h, int nh,
-                                                      int nseg)
+__global__ __launch_bounds__(kConvThreads) void conv_kernel(const TD *__restrict__ x, int64_t xstride,
+                                                      TD *__restrict__ y, int nt,
+                                                      const TD *__restrict__ h, int64_t hstride,
+                                                      int nh, int nseg)
 {
-    conv_segment<TD>(x, y, nt, h, nh, blockIdx.x / nseg, (blockIdx.x % nseg) * kConvOut);
+    conv_segment<TD>(x, xstride, y, nt, h, hstride, nh, blockIdx.x / nseg,
+                     (blockIdx.x % nseg) * kConvOut);
+}
+
+// y[r * nlag + j] = sum_u a[r * astride + u] b[r * bstride + lag0 + j + u]
+template <typename TD>
+__global__ __launch_bounds__(kXcorrThreads) void xcorr_kernel(const TD *__restrict__ a,
+                                                        int64_t astride, int la,
+                                                        const TD *__restrict__ b,
+                                                        int64_t bstride, int lb,
+                                                        TD *__restrict__ y, int nlag, int lag0)
+{
+    const int64_t r = blockIdx.x / nlag;
+    const int j = (int)(blockIdx.x % nlag);
+    const TD v = xcorr_lag<TD>(a + r * astride, la, b + r * bstride, lb, lag0 + j);
+    if (threadIdx.x == 0)
+        y[r * nlag + j] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -567,13 +584,16 @@ extern "C" int kx_gather(const kx_plan *p, const void *traces, void *m, void *st
     return launch_gather_d<float, float>(p, a, traces, m, st);
 }
 
-extern "C" int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const void *h, int32_t nh,
-                       int32_t dtype, void *stream)
+extern "C" int kx_conv_rows(const void *x, int64_t xstride, void *y, int64_t ntr, int32_t nt,
+                            const void *h, int64_t hstride, int32_t nh, int32_t dtype,
+                            void *stream)
 {
     if (!x || !y || !h || x == y)
         return kx_fail(KX_EINVAL, "conv: bad buffers");
     if (ntr <= 0 || nt <= 0 || nh <= 0 || (nh % 2) == 0)
         return kx_fail(KX_ESHAPE, "conv: need ntr, nt > 0 and odd nh (nh=%d)", nh);
+    if (xstride < 0 || hstride < 0)
+        return kx_fail(KX_ESHAPE, "conv: negative row stride");
     const int nseg = (nt + kConvOut - 1) / kConvOut;
     if (ntr * nseg > INT32_MAX)
         return kx_fail(KX_ESHAPE, "conv: too many traces");
@@ -584,15 +604,50 @@ extern "C" int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const vo
     const dim3 grid((unsigned)(ntr * nseg));
     if (dtype == KX_F32)
         hipLaunchKernelGGL(conv_kernel<float>, grid, dim3(kConvThreads), lds, st,
-                           reinterpret_cast<const float *>(x), reinterpret_cast<float *>(y), nt,
-                           reinterpret_cast<const float *>(h), nh, nseg);
+                           reinterpret_cast<const float *>(x), xstride, reinterpret_cast<float *>(y),
+                           nt, reinterpret_cast<const float *>(h), hstride, nh, nseg);
     else if (dtype == KX_F64)
         hipLaunchKernelGGL(conv_kernel<double>, grid, dim3(kConvThreads), lds, st,
-                           reinterpret_cast<const double *>(x), reinterpret_cast<double *>(y), nt,
-                           reinterpret_cast<const double *>(h), nh, nseg);
+                           reinterpret_cast<const double *>(x), xstride,
+                           reinterpret_cast<double *>(y), nt, reinterpret_cast<const double *>(h),
+                           hstride, nh, nseg);
     else
         return kx_fail(KX_ETYPE, "conv: dtype %d", dtype);
     KX_LAUNCHED("conv");
+    return KX_OK;
+}
+
+extern "C" int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const void *h, int32_t nh,
+                       int32_t dtype, void *stream)
+{
+    return kx_conv_rows(x, nt, y, ntr, nt, h, 0, nh, dtype, stream);
+}
+
+extern "C" int kx_xcorr_rows(const void *a, int64_t astride, int32_t la, const void *b,
+                             int64_t bstride, int32_t lb, void *y, int64_t ntr, int32_t nlag,
+                             int32_t lag0, int32_t dtype, void *stream)
+{
+    if (!a || !b || !y || y == a || y == b)
+        return kx_fail(KX_EINVAL, "xcorr: bad buffers");
+    if (ntr <= 0 || nlag <= 0 || la <= 0 || lb <= 0 || astride < 0 || bstride < 0)
+        return kx_fail(KX_ESHAPE, "xcorr: need ntr, nlag, la, lb > 0 and strides >= 0");
+    if (ntr * nlag > INT32_MAX)
+        return kx_fail(KX_ESHAPE, "xcorr: too many outputs");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)(ntr * nlag));
+    if (dtype == KX_F32)
+        hipLaunchKernelGGL(xcorr_kernel<float>, grid, dim3(kXcorrThreads), 0, st,
+                           reinterpret_cast<const float *>(a), astride, la,
+                           reinterpret_cast<const float *>(b), bstride, lb,
+                           reinterpret_cast<float *>(y), nlag, lag0);
+    else if (dtype == KX_F64)
+        hipLaunchKernelGGL(xcorr_kernel<double>, grid, dim3(kXcorrThreads), 0, st,
+                           reinterpret_cast<const double *>(a), astride, la,
+                           reinterpret_cast<const double *>(b), bstride, lb,
+                           reinterpret_cast<double *>(y), nlag, lag0);
+    else
+        return kx_fail(KX_ETYPE, "xcorr: dtype %d", dtype);
+    KX_LAUNCHED("xcorr");
     return KX_OK;
 }
 

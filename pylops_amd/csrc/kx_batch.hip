@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kConvThreads) void batch_conv_kernel(const BatchPro
     const int64_t k = b - cprefix[lo];
     const int nt = P.a.nt;
     const TD *h = reinterpret_cast<const TD *>(adjoint ? P.hstar : P.h);
-    conv_segment<TD>(x + P.doff, y + P.doff, nt, h, P.nh, k / P.nseg,
+    conv_segment<TD>(x + P.doff, nt, y + P.doff, nt, h, 0, P.nh, k / P.nseg,
                      (int)(k % P.nseg) * kConvOut);
 }
 

@@ -722,10 +722,13 @@ __host__ __device__ __forceinline__ int conv_lds_elems(int nh)
 }
 
 // Output samples [n0, n0 + kConvOut) of trace tr:
-// y[n] = sum_k h[k] x[n + (nh - 1) / 2 - k].
+// y[n] = sum_k h[k] x[n + (nh - 1) / 2 - k], with the trace's signal row
+// at x + tr * xstride and its filter at h + tr * hstride (a stride of 0
+// shares one row between all traces); y rows are nt apart.
 template <typename TD>
-__device__ __forceinline__ void conv_segment(const TD *__restrict__ x, TD *__restrict__ y, int nt,
-                                             const TD *__restrict__ h, int nh, int64_t tr, int n0)
+__device__ __forceinline__ void conv_segment(const TD *__restrict__ x, int64_t xstride,
+                                             TD *__restrict__ y, int nt, const TD *__restrict__ h,
+                                             int64_t hstride, int nh, int64_t tr, int n0)
 {
     extern __shared__ __align__(16) unsigned char kx_smem[];
     const int nhp = conv_taps_padded(nh);
@@ -734,9 +737,10 @@ __device__ __forceinline__ void conv_segment(const TD *__restrict__ x, TD *__res
     const int c = (nh - 1) / 2;
     const int start = n0 + c - (nh - 1);
     const int nwin = kConvOut + nhp + 4;
-    const TD *xr = x + tr * nt;
+    const TD *xr = x + tr * xstride;
+    const TD *hr = h + tr * hstride;
     for (int k = threadIdx.x; k < nhp; k += blockDim.x)
-        hs[k] = k < nh ? h[nh - 1 - k] : TD(0);  // flipped: out[i] = sum_k hs[k] win[i + k]
+        hs[k] = k < nh ? hr[nh - 1 - k] : TD(0);  // flipped: out[i] = sum_k hs[k] win[i + k]
     for (int j = threadIdx.x; j < nwin; j += blockDim.x) {
         const int n = start + j;
         win[j] = (n >= 0 && n < nt) ? xr[n] : TD(0);
@@ -773,6 +777,31 @@ __device__ __forceinline__ void conv_segment(const TD *__restrict__ x, TD *__res
         if (n0 + i0 + r < nt)
             y[tr * nt + n0 + i0 + r] = acc[r];
     __syncthreads();  // the window is read before a next item loads its own
+}
+
+// One lag of one trace: y = sum_{u < la} a[u] b[lag + u], b zero outside
+// [0, lb); summed in float64 by the workgroup (kXcorrThreads threads).
+constexpr int kXcorrThreads = 256;
+template <typename TD>
+__device__ __forceinline__ TD xcorr_lag(const TD *__restrict__ a, int la, const TD *__restrict__ b,
+                                        int lb, int lag)
+{
+    __shared__ double part[kXcorrThreads / kWave];
+    double acc = 0.0;
+    const int u0 = max(0, -lag), u1 = min(la, lb - lag);
+    for (int u = u0 + (int)threadIdx.x; u < u1; u += blockDim.x)
+        acc += (double)a[u] * (double)b[lag + u];
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1)
+        acc += __shfl_xor(acc, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        part[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w)
+        t += part[w];
+    __syncthreads();  // part is read before a next call writes it
+    return (TD)t;
 }
 
 }  // namespace

@@ -239,3 +239,30 @@ def test_segments_cover_each_segment_in_chunks():
         assert len(cs) >= 1 and all(seg[c] == p for c in cs)
         assert lo[first[p]] == off[p] and lo[first[p + 1]] == off[p + 1]
     assert S.work.numel() == S.nchunk
+
+
+def test_convolve1d_construction_rules():
+    """Convolve1D's host-side rules (C1D:21-47, 57-199): the method rule by
+    signal dimension, the offset bounds, the data shape of a long filter,
+    and filter shapes that do not broadcast."""
+    from pylops_amd.signalprocessing import Convolve1D
+    with pytest.raises(NotImplementedError):
+        Convolve1D(50, np.ones(5), method="overlapadd")
+    with pytest.raises(NotImplementedError):
+        Convolve1D((4, 50), np.ones(5), method="direct")
+    Convolve1D(50, np.ones(5), method="fft")
+    Convolve1D((4, 50), np.ones(5), method="overlapadd")
+    with pytest.raises(ValueError):
+        Convolve1D(50, np.ones(5), offset=5)
+    with pytest.raises(ValueError):
+        Convolve1D(10, np.ones(30), offset=10)
+    C = Convolve1D(10, np.ones(30), offset=9)
+    assert C.shape == (30, 10) and C.long
+    C = Convolve1D((3, 10), np.ones((3, 30)), axis=1)
+    assert C.dimsd == (3, 30) and C.dims == (3, 10)
+    with pytest.raises(ValueError):
+        Convolve1D((3, 10), np.ones(30), axis=1)  # 1-D long filter, 2-D signal
+    with pytest.raises(ValueError):
+        Convolve1D((3, 50), np.ones((2, 5)), axis=1)  # 2 filters for 3 traces
+    C = Convolve1D((3, 50), np.ones((1, 5)), axis=1)
+    assert C.h.shape == (3, 9)  # 5 taps padded for offset 0

@@ -340,3 +340,57 @@ def test_eikonal_second_order_tracks_the_contour_distance():
             assert med[2] < 3e-3 and e.max() < 2.5e-2, (med[2], e.max())
     assert med[1] > 5 * med[2], med
 
+
+# ---------------------------------------------------------------------------
+# Convolve1D with a filter longer than the signal (oracle.conv_long_*)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,offset", [(10, 0), (10, 4), (10, 9), (7, 0), (7, 3), (7, 6)])
+def test_conv_long_forward_is_scipy_same(n, offset):
+    """The forward is the reference's own call: fftconvolve(h, pad(x),
+    mode="same") (C1D:165-174), with the model padded by the offset rule."""
+    from scipy.signal import fftconvolve
+    rng = np.random.default_rng(n + 10 * offset)
+    x, h = rng.standard_normal(n), rng.standard_normal(31)
+    sh = 2 * (n // 2 - offset) - (1 if n % 2 == 0 else 0)
+    xp = np.pad(x, (max(sh, 0), -min(sh, 0)))
+    np.testing.assert_allclose(O.conv_long_forward(x, h, offset),
+                               fftconvolve(h, xp, mode="same"), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,offset", [(10, 0), (10, 4), (10, 9), (7, 0), (7, 3), (7, 6)])
+def test_conv_long_pair_is_adjoint(n, offset):
+    """The centre samples the adjoint keeps (C1D:188-199) make it the
+    adjoint of the forward: <A x, d> = <x, A^T d>."""
+    rng = np.random.default_rng(100 + n + offset)
+    h = rng.standard_normal(23)
+    x, d = rng.standard_normal(n), rng.standard_normal(23)
+    lhs = O.conv_long_forward(x, h, offset) @ d
+    rhs = x @ O.conv_long_adjoint(d, h, offset, n)
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs)), (lhs, rhs)
+
+
+def test_conv_long_delta_filter_shifts_the_model():
+    """A unit impulse at p as the long filter returns the padded model
+    shifted: y[i] = xp[i + (m - 1)//2 - p]."""
+    n, offset, nh, p = 8, 2, 40, 17
+    x = np.arange(1.0, n + 1)
+    h = np.zeros(nh)
+    h[p] = 1.0
+    sh = 2 * (n // 2 - offset) - 1
+    xp = np.pad(x, (max(sh, 0), -min(sh, 0)))
+    m = xp.size
+    want = np.array([xp[i + (m - 1) // 2 - p] if 0 <= i + (m - 1) // 2 - p < m else 0.0
+                     for i in range(nh)])
+    np.testing.assert_array_equal(O.conv_long_forward(x, h, offset), want)
+
+
+def test_conv_same_rows_matches_numpy_per_row():
+    """One filter per trace: each row is numpy's convolve, centred."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((4, 50))
+    h = rng.standard_normal((4, 9))
+    got = O.conv_same_rows(x, h)
+    for r in range(4):
+        np.testing.assert_allclose(got[r], np.convolve(x[r], h[r])[4:54], rtol=1e-12,
+                                   atol=1e-12)
+
