@@ -834,10 +834,11 @@ _THRESH = {"soft": soft_threshold, "hard": hard_threshold, "half": half_threshol
 
 
 def shrinkage(Op, y, alpha, niter=10, eps=0.1, tol=1e-10, threshkind="soft", x0=None,
-              fast=True, decay=None):
+              fast=True, decay=None, S=None):
     """ISTA (fast=False, cls_sparsity.py:1700-1800) or FISTA (fast=True,
-    cls_sparsity.py:2010-2120) with a given step ``alpha``; returns
-    ``(x, niter, cost)``."""
+    cls_sparsity.py:2010-2120) with a given step ``alpha``; ``S`` (a dense
+    matrix) is the analysis operator SOp: the threshold acts on S^H xu and
+    S maps back (cls_sparsity.py:1734-1758).  Returns ``(x, niter, cost)``."""
     thr = _THRESH[threshkind]
     thresh = eps * alpha * 0.5
     x = np.zeros(Op.shape[1]) if x0 is None else np.asarray(x0, dtype=np.float64).copy()
@@ -852,7 +853,10 @@ def shrinkage(Op, y, alpha, niter=10, eps=0.1, tol=1e-10, threshkind="soft", x0=
         res = y - Op.matvec(base)
         xu = base + alpha * Op.rmatvec(res)
         d = 1.0 if decay is None else decay[it]
-        x = thr(xu, d * thresh)
+        if S is None:
+            x = thr(xu, d * thresh)
+        else:
+            x = S @ thr(S.conj().T @ xu, d * thresh)
         if fast:
             told = t
             t = (1.0 + np.sqrt(1.0 + 4.0 * t ** 2)) / 2.0

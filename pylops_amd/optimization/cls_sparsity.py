@@ -8,8 +8,9 @@ monitor and cost history (FISTA's cost evaluates ``Op @ x`` as the
 reference does).  Vectors are torch tensors on the operator's device, the
 operator products are the HIP kernels; the step size, when not given, comes
 from the power iteration the reference uses on GPU arrays (10 iterations of
-``Op^H Op``, ``eigsdict`` passes ``niter`` / ``tol``).  The analysis form
-(``SOp``) is not provided.
+``Op^H Op``, ``eigsdict`` passes ``niter`` / ``tol``).  With ``SOp`` the
+threshold acts on ``SOp^H`` of the gradient-updated model and ``SOp`` maps
+the thresholded coefficients back (cls_sparsity.py:1734-1758, 2099-2123).
 """
 import logging
 import math
@@ -140,8 +141,10 @@ class ISTA(Solver):
               tol=1e-10, threshkind="soft", perc=None, decay=None, monitorres=False,
               show=False):
         """cls_sparsity.py:1460-1620; returns the initial model."""
-        if SOp is not None:
-            raise NotImplementedError("the analysis form (SOp) is not provided")
+        if SOp is not None and (SOp.shape[0] != self.Op.shape[1]):
+            raise ValueError(f"SOp of shape {SOp.shape} for a model of {self.Op.shape[1]} "
+                             "samples")
+        self.SOp = SOp
         if threshkind not in _THRESHKINDS:
             raise NotImplementedError("threshkind should be hard, soft, half,"
                                       "hard-percentile, soft-percentile, "
@@ -182,10 +185,20 @@ class ISTA(Solver):
         return x
 
     def _threshold(self, xu):
+        """Threshold the gradient-updated model, or its SOp^H projection
+        followed by SOp (cls_sparsity.py:1734-1758)."""
+        if self.SOp is not None:
+            c = torch.empty(self.SOp.shape[1], dtype=xu.dtype, device=xu.device)
+            xu = _apply_into(self.SOp, xu, c, True)
         if self.perc is None:
             d = 1.0 if self.decay is None else float(self.decay[self.iiter])
-            return self.threshf(xu, d * self.thresh)
-        return self.threshf(xu, 100 - self.perc)
+            x = self.threshf(xu, d * self.thresh)
+        else:
+            x = self.threshf(xu, 100 - self.perc)
+        if self.SOp is not None:
+            out = torch.empty(self.SOp.shape[0], dtype=x.dtype, device=x.device)
+            x = _apply_into(self.SOp, x, out, False)
+        return x
 
     def _monitor(self, res):
         if self.monitorres:

@@ -394,3 +394,17 @@ def test_conv_same_rows_matches_numpy_per_row():
         np.testing.assert_allclose(got[r], np.convolve(x[r], h[r])[4:54], rtol=1e-12,
                                    atol=1e-12)
 
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_shrinkage_analysis_form_with_orthonormal_s(fast):
+    """With an orthonormal S the analysis form on A is the synthesis form
+    on A S mapped back by S: x = S c (a transposed S fails this)."""
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((30, 20))
+    S, _ = np.linalg.qr(rng.standard_normal((20, 20)))
+    y = rng.standard_normal(30)
+    alpha = 1.0 / np.linalg.eigvalsh(A.T @ A).max()
+    xa, _, _ = O.shrinkage(O.DenseOp(A), y, alpha, niter=15, eps=0.3, tol=0, fast=fast, S=S)
+    c, _, _ = O.shrinkage(O.DenseOp(A @ S), y, alpha, niter=15, eps=0.3, tol=0, fast=fast)
+    np.testing.assert_allclose(xa, S @ c, rtol=1e-10, atol=1e-12)
