@@ -19,23 +19,28 @@ def _device_of(Op, y):
     return dev if dev is not None else torch.device("cuda", torch.cuda.current_device())
 
 
-def _to_dev(a, device, dtype):
+def _to_dev(a, device, dtype, flat=True):
     if a is None:
         return None
     if is_torch(a):
-        return a.to(device=device, dtype=dtype).reshape(-1).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(np.asarray(a).ravel()), dtype=dtype,
+        a = a.to(device=device, dtype=dtype)
+        return (a.reshape(-1) if flat else a).contiguous()
+    a = np.asarray(a)
+    return torch.as_tensor(np.ascontiguousarray(a.ravel() if flat else a), dtype=dtype,
                            device=device)
 
 
-def _prepare(Op, y, x0, callback, solver):
+def _prepare(Op, y, x0, callback, solver, flat=True):
+    """y and x0 on the operator's device (raveled unless ``flat`` is False),
+    the callback wrapped for numpy callers, and the map back to the
+    caller's array type."""
     from pylops_amd import _lib
     _lib.require_device()
     dev = _device_of(Op, y)
     dt = y.dtype if is_torch(y) else torch_dtype(np.asarray(y).dtype)
     numpy_io = not is_torch(y)
-    yd = _to_dev(y, dev, dt)
-    x0d = _to_dev(x0, dev, dt)
+    yd = _to_dev(y, dev, dt, flat)
+    x0d = _to_dev(x0, dev, dt, flat)
     if callback is not None:
         if numpy_io:
             solver.callback = lambda x: callback(x.detach().cpu().numpy())

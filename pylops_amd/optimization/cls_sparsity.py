@@ -118,7 +118,7 @@ class ISTA(Solver):
         print("   Itn          x[0]        r2norm     r12norm     xupdate")
 
     def _print_step(self, x, costdata, costreg, xupdate):
-        print(f"{self.iiter:6g} {float(x[0]):12.5e}  {costdata:10.3e}   "
+        print(f"{self.iiter:6g} {float(x.reshape(-1)[0]):12.5e}  {costdata:10.3e}   "
               f"{costdata + costreg:9.3e}  {xupdate:10.3e}")
 
     def memory_usage(self, show=False, unit="B"):
@@ -129,13 +129,24 @@ class ISTA(Solver):
                   f"{memuse / _units[unit]:.2f} {unit}")
         return memuse
 
+    @staticmethod
+    def _apply(Op, x, adjoint):
+        """Op x (Op^H x if adjoint) for a vector, or Op.matmat / rmatmat for
+        the columns of a matrix (several right-hand sides,
+        cls_sparsity.py:1544-1557)."""
+        if x.ndim == 2:
+            y = Op.rmatmat(x) if adjoint else Op.matmat(x)
+            return torch.as_tensor(y, dtype=x.dtype, device=x.device).reshape(
+                Op.shape[1] if adjoint else Op.shape[0], x.shape[1])
+        out = torch.empty(Op.shape[1] if adjoint else Op.shape[0], dtype=x.dtype,
+                          device=x.device)
+        return _apply_into(Op, x, out, adjoint)
+
     def _fwd(self, x):
-        out = torch.empty(self.Op.shape[0], dtype=x.dtype, device=x.device)
-        return _apply_into(self.Op, x, out, False)
+        return self._apply(self.Op, x, False)
 
     def _adj(self, r):
-        out = torch.empty(self.Op.shape[1], dtype=r.dtype, device=r.device)
-        return _apply_into(self.Op, r, out, True)
+        return self._apply(self.Op, r, True)
 
     def setup(self, y, x0=None, niter=None, SOp=None, eps=0.1, alpha=None, eigsdict=None,
               tol=1e-10, threshkind="soft", perc=None, decay=None, monitorres=False,
@@ -172,9 +183,14 @@ class ISTA(Solver):
             self.alpha = 1.0 / maxeig
         self.thresh = eps * self.alpha * 0.5
         if x0 is None:
-            x = torch.zeros(self.Op.shape[1], dtype=dt, device=dev)
+            shape = (self.Op.shape[1],) if y.ndim == 1 else (self.Op.shape[1], y.shape[1])
+            x = torch.zeros(shape, dtype=dt, device=dev)
         else:
-            x = x0.reshape(-1).to(device=dev, dtype=dt).clone()
+            if y.ndim != x0.ndim:
+                raise ValueError("Number of columns of x0 and data are not the same")
+            if x0.shape[0] != self.Op.shape[1]:
+                raise ValueError("Operator and input vector have different dimensions")
+            x = x0.to(device=dev, dtype=dt).clone()
         if monitorres:
             self.normresold = np.inf
         self.t = 1.0
@@ -188,16 +204,14 @@ class ISTA(Solver):
         """Threshold the gradient-updated model, or its SOp^H projection
         followed by SOp (cls_sparsity.py:1734-1758)."""
         if self.SOp is not None:
-            c = torch.empty(self.SOp.shape[1], dtype=xu.dtype, device=xu.device)
-            xu = _apply_into(self.SOp, xu, c, True)
+            xu = self._apply(self.SOp, xu, True)
         if self.perc is None:
             d = 1.0 if self.decay is None else float(self.decay[self.iiter])
             x = self.threshf(xu, d * self.thresh)
         else:
             x = self.threshf(xu, 100 - self.perc)
         if self.SOp is not None:
-            out = torch.empty(self.SOp.shape[0], dtype=x.dtype, device=x.device)
-            x = _apply_into(self.SOp, x, out, False)
+            x = self._apply(self.SOp, x, False)
         return x
 
     def _monitor(self, res):

@@ -149,3 +149,38 @@ def test_analysis_form_sop_matches_oracle(fast):
     assert it == ito == 25
     assert G.relerr(np.asarray(x).ravel(), xo) < 1e-9
     np.testing.assert_allclose(cost, costo, rtol=1e-9)
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_several_right_hand_sides(fast):
+    """y with several columns (cls_sparsity.py:1544-1557, 1633-1646): the
+    products go through matmat / rmatmat, every column follows its own
+    single right-hand-side iterations (same alpha, elementwise thresholds),
+    and the cost uses the matrix norms as numpy's: 0.5 |R|_F^2 + eps |X|_1
+    (the largest column l1 norm)."""
+    import pylops_amd
+    from pylops_amd.optimization.sparsity import fista, ista
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dynamic=True, dtype="float64")
+    Op = pylops_amd.Kirchhoff(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    n = Op.shape[1]
+    A = np.stack([orc.matvec(e) for e in np.eye(n)], axis=1)
+    rng = np.random.default_rng(12)
+    M = np.zeros((n, 3))
+    for j in range(3):
+        M[rng.choice(n, 8, replace=False), j] = rng.uniform(-1, 1, 8)
+    Y = A @ M
+    alpha = 1.0 / np.linalg.eigvalsh(A.T @ A).max()
+    eps = 1e-2 * float(np.abs(A.T @ Y).max())
+    f = fista if fast else ista
+    X, it, cost = f(Op, Y, niter=20, eps=eps, alpha=alpha, tol=0)
+    assert X.shape == (n, 3) and it == 20
+    for j in range(3):
+        xj = f(Op, Y[:, j], niter=20, eps=eps, alpha=alpha, tol=0)[0]
+        np.testing.assert_allclose(X[:, j], xj, rtol=1e-11, atol=1e-14)
+    if fast:
+        want = 0.5 * np.linalg.norm(Y - A @ X) ** 2 + eps * np.linalg.norm(X, ord=1)
+        np.testing.assert_allclose(cost[-1], want, rtol=1e-9)
+    with pytest.raises(ValueError):
+        f(Op, Y, x0=np.zeros(n), niter=2, eps=eps, alpha=alpha)
