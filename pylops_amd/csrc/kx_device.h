@@ -187,23 +187,59 @@ template <int RB, bool ROT> __device__ __forceinline__ int step_receiver(int lan
     return ROT ? (lane + r) & (RB - 1) : r;
 }
 
+// Receiver bases held per lane: the base of receiver (lane mod RB) of the
+// block, one load for all RB receivers, handed to the lanes that need it by
+// a lane read (natural order) or a quad permutation (rotated order, RB <= 4);
+// rotated blocks of 8 and 16 receivers load one base per step instead.
+template <bool ROT, int RB> struct BaseLoads {
+    static constexpr int n = (ROT && RB >= 8) ? RB : 1;
+};
+
 // One lane's share of one tile: the image value and the table entries of
 // its point, receivers in the lane's step order.
-template <typename TD, typename TT, int RB> struct SpreadTile {
+template <typename TD, typename TT, bool ROT, int RB> struct SpreadTile {
     TD mv;
     TT os;       // raw table elements: converted at use, so that a prefetch
     TT orr[RB];  // never waits for them
     TD ps, qs;
     TD pr[RB], qr[RB];
     int bs;
-    int br[RB];
+    int br[BaseLoads<ROT, RB>::n];
 };
+
+// The receiver base of step r for this lane.
+template <bool ROT, int RB>
+__device__ __forceinline__ int step_base(const int (&br)[BaseLoads<ROT, RB>::n], int r)
+{
+    if constexpr (BaseLoads<ROT, RB>::n > 1) {
+        return br[r];
+    } else if constexpr (!ROT || RB == 1) {
+        // every lane treats receiver r: the base held by lane r
+        return __builtin_amdgcn_readlane(br[0], r);
+    } else if constexpr (RB == 2) {
+        // lane l treats receiver (l + r) mod 2: swap lane pairs on odd steps
+        return (r & 1) ? __builtin_amdgcn_update_dpp(0, br[0], 0xB1, 0xf, 0xf, false) : br[0];
+    } else {
+        // RB == 4: lane l treats receiver (l + r) mod 4, held by lane
+        // (l & ~3) | ((l + r) & 3): quad_perm [r, r+1, r+2, r+3] mod 4
+        switch (r & 3) {
+        case 1:
+            return __builtin_amdgcn_update_dpp(0, br[0], 0x39, 0xf, 0xf, false);
+        case 2:
+            return __builtin_amdgcn_update_dpp(0, br[0], 0x4E, 0xf, 0xf, false);
+        case 3:
+            return __builtin_amdgcn_update_dpp(0, br[0], 0x93, 0xf, 0xf, false);
+        default:
+            return br[0];
+        }
+    }
+}
 
 template <typename TD, typename TT, bool DYN, bool ROT, int RB>
 __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c, int lane, int rel,
                                             int lz, int lx, int ly, int s, int r0, int nrl,
                                             const TD *__restrict__ m, int vz,
-                                            SpreadTile<TD, TT, RB> &L)
+                                            SpreadTile<TD, TT, ROT, RB> &L)
 {
     const int t = c.t;
     const int iz = c.tzi * a.tz + lz, ix = c.txi * a.tx + lx, iy = c.tyi * a.ty + ly;
@@ -222,13 +258,18 @@ __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c,
     }
     const TT *orow = reinterpret_cast<const TT *>(a.off_r) + tr * kWave;
     const int *brow = a.base_r + tr;
+    if constexpr (BaseLoads<ROT, RB>::n == 1) {
+        const int jl = lane & (RB - 1);
+        L.br[0] = brow[(unsigned)((jl < nrl ? jl : 0) + vz)];
+    }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         int j = step_receiver<RB, ROT>(lane, r);
         j = j < nrl ? j : 0;  // rows past the last receiver read row 0
         const unsigned e = (unsigned)(j * kWave + lane);
         L.orr[r] = orow[e];
-        L.br[r] = brow[(unsigned)(j + vz)];
+        if constexpr (BaseLoads<ROT, RB>::n > 1)
+            L.br[r] = brow[(unsigned)(j + vz)];
         if constexpr (DYN) {
             L.pr[r] = (reinterpret_cast<const TD *>(a.wp_r) + tr * kWave)[e];
             L.qr[r] = (reinterpret_cast<const TD *>(a.wq_r) + tr * kWave)[e];
@@ -259,8 +300,9 @@ __device__ __forceinline__ int floor_int(float x)
 // subtraction rounds it, except that a result that would round to 1 (T in
 // (-2^-24, 0)) stays just below 1.  Dynamic weights are
 // (m ps) pr + (m qs) qr with the source-side products formed once per tile.
-template <typename TD, typename TT, bool DYN, bool NEG, bool FIX, int RB>
-__device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, RB> &cur, int iz, int e,
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
+__device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, RB> &cur, int iz,
+                                                  int e,
                                                   const int (&lrow)[RB], const int (&izm)[RB],
                                                   unsigned ntm1, double *accd,
                                                   unsigned long long *acci)
@@ -281,15 +323,16 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, RB> &
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         const TC T = os + (TC)ld_off(&cur.orr[r]);
+        const int br = step_base<ROT, RB>(cur.br, r);
         TC f;
         int k;
         if constexpr (!NEG && sizeof(TC) == 4) {
             f = __builtin_amdgcn_fractf(T);
-            k = floor_int(T) + cur.bs + cur.br[r];
+            k = floor_int(T) + cur.bs + br;
         } else {
             const TC fl = floor(T);
             f = T - fl;
-            k = (int)fl + cur.bs + cur.br[r];
+            k = (int)fl + cur.bs + br;
             if constexpr (NEG) {
                 // the reference truncates toward zero: a time in (-1, 0)
                 // lands on sample 0 with a negative weight
@@ -359,7 +402,7 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     // the next tile is loaded while this one is accumulated (a twice
     // unrolled loop with two buffers and no copies measured slower: the
     // second buffer costs registers and occupancy)
-    SpreadTile<TD, TT, RB> cur, nxt;
+    SpreadTile<TD, TT, ROT, RB> cur, nxt;
     TileCursor c;
     c.start(a, wave);
     if (c.t < a.ntiles)
@@ -369,8 +412,8 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             spread_load<TD, TT, DYN, ROT, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
-        spread_accumulate<TD, TT, DYN, NEG, FIX, RB>(cur, c.tzi * a.tz + lz, e, lrow, izm, ntm1,
-                                                     accd, acci);
+        spread_accumulate<TD, TT, DYN, NEG, ROT, FIX, RB>(cur, c.tzi * a.tz + lz, e, lrow, izm,
+                                                          ntm1, accd, acci);
         cur = nxt;
         c = cn;
     }
