@@ -1,7 +1,7 @@
 """Write the golden input/output vectors under tests/golden/.
 
-Run once, in the build container only (the reference is not available on the
-GPU box):  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
+Run once, where the reference sources are present (they are not part of this
+repository):  PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
 
 Only behaviour that no closed form, invariant or library routine pins is
 recorded here (the reference-specific quirks):
