@@ -136,6 +136,14 @@ int kx_tile_table(const kx_plan *geom, const double *tabT, int32_t n,
 int kx_spread(const kx_plan *p, const void *m, void *traces, void *stream);
 int kx_gather(const kx_plan *p, const void *traces, void *m, void *stream);
 
+/* kx_spread for nmodel models at once (a float32 operator; 1 <= nmodel <=
+ * 64): model j at m + j*ni, its traces at traces + j*ns*nr*nt.  The
+ * column-by-column forward of LinearOperator.matmat (LO:436-447) with the
+ * table loads and index arithmetic of a tile shared by 2, 4 or 8 models per
+ * workgroup; each model's traces are bitwise its kx_spread.  Uses plan.aux
+ * for the per-model scales (bytes [2048, 2048 + 8*nmodel)). */
+int kx_spread_models(const kx_plan *p, const void *m, void *traces, int32_t nmodel, void *stream);
+
 /* "same"-mode convolution of each of ntr rows of length nt with an odd
  * filter h[nh]:  y[n] = sum_k h[k] x[n + (nh-1)/2 - k].  x != y. */
 int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const void *h,

@@ -96,6 +96,25 @@ __global__ __launch_bounds__(kSpreadBlock) void spread_kernel(KArgs a, const TD 
     spread_item<TD, TT, DYN, NEG, ROT, RB>(a, m, y, scale, L % a.ns, (L / a.ns) * RB);
 }
 
+// K models of a float32 operator; workgroup L (after the XCD renumbering) =
+// source L mod ns, receiver L / ns.  The launch accumulates in fixed point
+// when every model's scale allows it, in float64 otherwise.
+template <typename TT, bool DYN, bool NEG, int K>
+__global__ __launch_bounds__(kSpreadBlock) void spread_models_kernel(
+    KArgs a, const float *__restrict__ m, int64_t ni, float *__restrict__ y, int64_t ystride,
+    const SpreadScale *__restrict__ sc)
+{
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    bool fixed = true;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        fixed = fixed && sc[j].fixed;
+    if (fixed)
+        spread_models_body<TT, DYN, NEG, true, K>(a, m, ni, y, ystride, sc, L % a.ns, L / a.ns);
+    else
+        spread_models_body<TT, DYN, NEG, false, K>(a, m, ni, y, ystride, sc, L % a.ns, L / a.ns);
+}
+
 template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
 __global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD *__restrict__ d,
                                                               TD *__restrict__ m)
@@ -566,6 +585,98 @@ extern "C" int kx_spread(const kx_plan *p, const void *m, void *traces, void *st
     if (p->tab_fmt == KX_F16)
         return launch_spread_d<float, __half>(p, a, m, traces, st);
     return launch_spread_d<float, float>(p, a, m, traces, st);
+}
+
+namespace {
+
+constexpr int kMaxModels = 64;
+constexpr size_t kModelScaleOffset = 2048;  // bytes into plan.aux
+
+template <typename TT, bool DYN, bool NEG, int K>
+void launch_models_k(const KArgs &a, const float *m, int64_t ni, float *y, int64_t ystride,
+                     const SpreadScale *sc, hipStream_t st)
+{
+    const size_t lds = spread_lds_bytes(K, a.nt);
+    hipLaunchKernelGGL((spread_models_kernel<TT, DYN, NEG, K>), dim3(a.ns * a.nr),
+                       dim3(kSpreadBlock), lds, st, a, m, ni, y, ystride, sc);
+}
+
+template <typename TT, bool DYN, bool NEG>
+int launch_models(const kx_plan *p, const KArgs &a, const float *m, float *y, int nmodel,
+                  hipStream_t st)
+{
+    const int64_t ni = (int64_t)p->nx * p->ny * p->nz;
+    const int64_t ystride = (int64_t)p->ns * p->nr * p->nt;
+    SpreadScale *scs = reinterpret_cast<SpreadScale *>(static_cast<char *>(p->aux) +
+                                                       kModelScaleOffset);
+    float *part = reinterpret_cast<float *>(p->aux) + 16;
+    const int nb = (int)std::min<int64_t>(kScaleBlocks, (ni + kBlock - 1) / kBlock);
+    for (int j = 0; j < nmodel; ++j) {
+        hipLaunchKernelGGL(absmax_partial_kernel, dim3(nb), dim3(kBlock), 0, st, m + j * ni, ni,
+                           part);
+        hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp,
+                           scs + j);
+    }
+    KX_LAUNCHED("spread models scale");
+    int j = 0;
+    while (j < nmodel) {
+        const int left = nmodel - j;
+        auto fits = [&](int k) {
+            return k <= left && spread_lds_bytes(k, a.nt) <= (size_t)kLdsBudget;
+        };
+        const float *mj = m + j * ni;
+        float *yj = y + j * ystride;
+        if (fits(8)) {
+            launch_models_k<TT, DYN, NEG, 8>(a, mj, ni, yj, ystride, scs + j, st);
+            j += 8;
+        } else if (fits(4)) {
+            launch_models_k<TT, DYN, NEG, 4>(a, mj, ni, yj, ystride, scs + j, st);
+            j += 4;
+        } else if (fits(2)) {
+            launch_models_k<TT, DYN, NEG, 2>(a, mj, ni, yj, ystride, scs + j, st);
+            j += 2;
+        } else {
+            const int rc = launch_spread_d<float, TT>(p, a, mj, yj, st);
+            if (rc)
+                return rc;
+            j += 1;
+        }
+        KX_LAUNCHED("spread models");
+    }
+    return KX_OK;
+}
+
+}  // namespace
+
+extern "C" int kx_spread_models(const kx_plan *p, const void *m, void *traces, int32_t nmodel,
+                                void *stream)
+{
+    int rc = check_plan(p);
+    if (rc)
+        return rc;
+    if (!m || !traces)
+        return kx_fail(KX_EINVAL, "spread_models: NULL buffer");
+    if (p->dtype != KX_F32)
+        return kx_fail(KX_ETYPE, "spread_models: float32 operators only");
+    if (nmodel < 1 || nmodel > kMaxModels)
+        return kx_fail(KX_ESHAPE, "spread_models: nmodel=%d outside [1, %d]", nmodel, kMaxModels);
+    if (!p->aux)
+        return kx_fail(KX_EINVAL, "spread_models: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
+    if (spread_lds_bytes(1, p->nt) > (size_t)kLdsBudget)
+        return kx_fail(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const KArgs a = make_args(p);
+    const float *mf = reinterpret_cast<const float *>(m);
+    float *yf = reinterpret_cast<float *>(traces);
+    if (p->tab_fmt == KX_F16)
+        return p->dynamic ? (p->neg_trav ? launch_models<__half, true, true>(p, a, mf, yf, nmodel, st)
+                                         : launch_models<__half, true, false>(p, a, mf, yf, nmodel, st))
+                          : (p->neg_trav ? launch_models<__half, false, true>(p, a, mf, yf, nmodel, st)
+                                         : launch_models<__half, false, false>(p, a, mf, yf, nmodel, st));
+    return p->dynamic ? (p->neg_trav ? launch_models<float, true, true>(p, a, mf, yf, nmodel, st)
+                                     : launch_models<float, true, false>(p, a, mf, yf, nmodel, st))
+                      : (p->neg_trav ? launch_models<float, false, true>(p, a, mf, yf, nmodel, st)
+                                     : launch_models<float, false, false>(p, a, mf, yf, nmodel, st));
 }
 
 extern "C" int kx_gather(const kx_plan *p, const void *traces, void *m, void *stream)

@@ -444,6 +444,161 @@ __device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// forward for K models at once (float32 operators)
+// ---------------------------------------------------------------------------
+// One workgroup = one (source, receiver) trace of K models: the tables of a
+// tile are loaded once and the index, fraction and aperture gate of every
+// point computed once for all K models; model j accumulates into LDS row j
+// with its own fixed-point scale.  Each model's contributions are the
+// integers its single-model launch adds, so every model's traces are bitwise
+// those of kx_spread (the sums are exact).
+template <typename TT, int K> struct ModelsTile {
+    float mv[K];
+    TT os, orr;
+    float ps, qs, pr, qr;
+    int bs, br;
+};
+
+template <typename TT, bool DYN, int K>
+__device__ __forceinline__ void models_load(const KArgs &a, const TileCursor &c, int lane, int rel,
+                                            int lz, int lx, int ly, int s, int r,
+                                            const float *__restrict__ m, int64_t ni, int vz,
+                                            ModelsTile<TT, K> &L)
+{
+    const int t = c.t;
+    const int iz = c.tzi * a.tz + lz, ix = c.txi * a.tx + lx, iy = c.tyi * a.ty + ly;
+    const bool valid = (iz < a.nz) & (ix < a.nx) & (iy < a.ny);
+    const int orig = (c.tyi * a.ty * a.nx + c.txi * a.tx) * a.nz + c.tzi * a.tz;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        L.mv[j] = valid ? (m + j * ni + orig)[(unsigned)rel] : 0.f;
+    const size_t ts = (size_t)t * a.ns + s;
+    const size_t tr = (size_t)t * a.nr + r;
+    L.os = (reinterpret_cast<const TT *>(a.off_s) + ts * kWave)[(unsigned)lane];
+    L.orr = (reinterpret_cast<const TT *>(a.off_r) + tr * kWave)[(unsigned)lane];
+    L.bs = a.base_s[ts + vz];
+    L.br = a.base_r[tr + vz];
+    if constexpr (DYN) {
+        L.ps = (reinterpret_cast<const float *>(a.wp_s) + ts * kWave)[(unsigned)lane];
+        L.qs = (reinterpret_cast<const float *>(a.wq_s) + ts * kWave)[(unsigned)lane];
+        L.pr = (reinterpret_cast<const float *>(a.wp_r) + tr * kWave)[(unsigned)lane];
+        L.qr = (reinterpret_cast<const float *>(a.wq_r) + tr * kWave)[(unsigned)lane];
+    }
+}
+
+template <typename TT, bool DYN, bool NEG, bool FIX, int K>
+__device__ __forceinline__ void models_accumulate(const ModelsTile<TT, K> &cur, int iz,
+                                                  const int (&e)[K], int izm, unsigned ntm1,
+                                                  int nt, double *accd, unsigned long long *acci)
+{
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        any = any || cur.mv[j] != 0.f;
+    if (__ballot(any) == 0)
+        return;
+    const float T = ld_off(&cur.os) + ld_off(&cur.orr);
+    float f;
+    int k;
+    if constexpr (!NEG) {
+        f = __builtin_amdgcn_fractf(T);
+        k = floor_int(T) + cur.bs + cur.br;
+    } else {
+        const float fl = floorf(T);
+        f = T - fl;
+        k = (int)fl + cur.bs + cur.br;
+        if (k == -1 && f > 0.f) {
+            k = 0;
+            f -= 1.f;
+        }
+    }
+    bool on = (unsigned)k < ntm1;
+    if constexpr (DYN)
+        on = on && iz >= izm;
+    if (!on)
+        return;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        float mv = cur.mv[j];
+        if constexpr (FIX)
+            mv = ldexpf(mv, e[j]);
+        float w = mv;
+        if constexpr (DYN)
+            w = fmaf(mv * cur.qs, cur.qr, (mv * cur.ps) * cur.pr);
+        if (w != 0.f) {
+            const int it = j * nt + k;
+            const float w1 = w * f;
+            if constexpr (FIX) {
+                const float w0 = fmaf(-w, f, w);
+                atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
+                atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
+            } else {
+                const float w0 = w * (1.f - f);
+                atomicAdd(&accd[it], (double)w0);
+                atomicAdd(&accd[it + 1], (double)w1);
+            }
+        }
+    }
+}
+
+// Traces (s, r) of models [0, K): m + j*ni is model j, y + j*ystride its
+// traces; sc[j] its fixed-point scale.
+template <typename TT, bool DYN, bool NEG, bool FIX, int K>
+__device__ __forceinline__ void spread_models_body(const KArgs &a, const float *__restrict__ m,
+                                                   int64_t ni, float *__restrict__ y,
+                                                   int64_t ystride,
+                                                   const SpreadScale *__restrict__ sc, int s,
+                                                   int r)
+{
+    extern __shared__ __align__(16) unsigned char kx_smem[];
+    double *accd = reinterpret_cast<double *>(kx_smem);
+    unsigned long long *acci = reinterpret_cast<unsigned long long *>(kx_smem);
+    const int ntr = K * a.nt;
+    for (int i = threadIdx.x; i < ntr; i += blockDim.x)
+        acci[i] = 0ull;
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nw = blockDim.x / kWave;
+    const int vz = vgpr_zero();
+    const int lz = lane % a.tz, lx = (lane / a.tz) % a.tx, ly = lane / (a.tz * a.tx);
+    const int rel = (ly * a.nx + lx) * a.nz + lz;
+    int e[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        e[j] = sc[j].e;
+    const int izm = DYN ? a.izmin[s * a.nr + r] : 0;
+    const unsigned ntm1 = (unsigned)(a.nt - 1);
+
+    ModelsTile<TT, K> cur, nxt;
+    TileCursor c;
+    c.start(a, wave);
+    if (c.t < a.ntiles)
+        models_load<TT, DYN, K>(a, c, lane, rel, lz, lx, ly, s, r, m, ni, vz, cur);
+    while (c.t < a.ntiles) {
+        TileCursor cn = c;
+        cn.advance(a, nw);
+        if (cn.t < a.ntiles)
+            models_load<TT, DYN, K>(a, cn, lane, rel, lz, lx, ly, s, r, m, ni, vz, nxt);
+        models_accumulate<TT, DYN, NEG, FIX, K>(cur, c.tzi * a.tz + lz, e, izm, ntm1, a.nt,
+                                                accd, acci);
+        cur = nxt;
+        c = cn;
+    }
+    __syncthreads();
+    const size_t off = ((size_t)s * a.nr + r) * a.nt;
+    for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
+        const int j = i / a.nt, q = i - j * a.nt;
+        float *yo = y + j * ystride + off;
+        if constexpr (FIX)
+            yo[q] = (float)ldexp((double)(long long)acci[i], -e[j]);
+        else
+            yo[q] = (float)accd[i];
+    }
+}
+
 // Per-launch scale of the fixed-point accumulation: partial max|m| per block
 // (inf for a non-finite value), then one wave turns the maximum into e.
 // Block b of nb strides over m[0, n); the result is valid in thread 0.

@@ -136,3 +136,61 @@ def test_torchoperator_batch_kirchhoff_vs_oracle():
     np.testing.assert_array_equal(g, Op.rmatmat(v.T).T)
     for j in range(4):
         assert G.relerr(g[j], orc.rmatvec(v[j].astype(np.float64))) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["dyn_f32", "dyn_f16", "kin_f32", "kin3d_f16", "neg_byot"])
+@pytest.mark.parametrize("k", [2, 8, 13])
+def test_matmat_forward_models_bitwise(case, k):
+    """The float32 matmat forward spreads 8, 4 or 2 models per workgroup
+    (kx_spread_models): every column is bitwise its matvec, for dynamic and
+    kinematic operators, float16 tables, 3D, and the negative-time path."""
+    import pylops_amd
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dtype="float32")
+    if case == "dyn_f16":
+        kw.update(dynamic=True, tabledtype="float16")
+    elif case == "dyn_f32":
+        kw.update(dynamic=True)
+    elif case == "kin3d_f16":
+        kw.update(y=G.y, srcs=G.s3d, recs=G.r3d, tabledtype="float16")
+    elif case == "neg_byot":
+        ts, tr, _, _, _, _ = pylops_amd.Kirchhoff._traveltime_table(G.z, G.x, G.s2d, G.r2d,
+                                                                    G.v0, mode="analytic")
+        kw.update(mode="byot", trav=(ts - 0.01, tr), vel=None)
+    Op = pylops_amd.Kirchhoff(**kw)
+    rng = np.random.default_rng(k)
+    X = rng.standard_normal((Op.shape[1], k)).astype(np.float32)
+    X[:, 1] = 0.0  # an all-zero model
+    Y = Op.matmat(X)
+    for j in range(k):
+        np.testing.assert_array_equal(Y[:, j], Op.matvec(X[:, j]))
+
+
+def test_matmat_forward_models_nonfinite_column():
+    """A non-finite model switches its launch to float64 accumulation, as
+    kx_spread does: the NaN propagates to that column's traces, the other
+    columns stay within rounding of their matvecs."""
+    Op = _kop("float32")
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((Op.shape[1], 4)).astype(np.float32)
+    X[7, 2] = np.nan
+    Y = Op.matmat(X)
+    for j in range(4):
+        want = Op.matvec(X[:, j])
+        np.testing.assert_array_equal(np.isnan(Y[:, j]), np.isnan(want))
+        ok = ~np.isnan(want)
+        assert G.relerr(Y[ok, j], want[ok]) < 1e-6
+
+
+def test_spread_models_rejects_float64():
+    """kx_spread_models is float32-only: a float64 plan is refused (the
+    float64 matmat runs the columns through kx_spread)."""
+    import ctypes
+    import torch
+    from pylops_amd import _lib
+    Op = _kop("float64")
+    m = torch.zeros((2, Op.shape[1]), dtype=torch.float64, device="cuda")
+    y = torch.empty((2, Op.shape[0]), dtype=torch.float64, device="cuda")
+    rc = _lib.load().kx_spread_models(ctypes.byref(Op.plan), _lib.ptr(m), _lib.ptr(y), 2,
+                                      _lib.stream_ptr(torch.device("cuda")))
+    assert rc == _lib.KX_ETYPE
