@@ -624,7 +624,7 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
 
     def data_term(m):
         r = Top.apply(m) / sig - dobs
-        return 0.5 * (r.double() ** 2).sum()
+        return 0.5 * torch.sum(r * r, dtype=torch.float64)
 
     def loss_fn(m):
         return data_term(m) + lam * isotropic_tv(m.view(-1, nx, nz)).double()
@@ -646,7 +646,10 @@ def autograd_line(dev, args, nbatch=16, nsteps=50, ndir=32):
         pred = Top.apply(m)
         ev[k][1].record()
         r = pred / sig - dobs
-        loss = 0.5 * (r.double() ** 2).sum() + lam * isotropic_tv(m.view(-1, nx, nz)).double()
+        # squares in float32, summed in float64 (no float64 copy of the
+        # 16 x 27.6M residual samples)
+        loss = (0.5 * torch.sum(r * r, dtype=torch.float64)
+                + lam * isotropic_tv(m.view(-1, nx, nz)).double())
         loss.backward()
         ev[k][2].record()
         opt.step()
