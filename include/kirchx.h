@@ -144,6 +144,15 @@ int kx_gather(const kx_plan *p, const void *traces, void *m, void *stream);
  * for the per-model scales (bytes [2048, 2048 + 8*nmodel)). */
 int kx_spread_models(const kx_plan *p, const void *m, void *traces, int32_t nmodel, void *stream);
 
+/* kx_gather for nmodel models at once (a float32 operator): traces of
+ * model j at traces + j*ns*nr*nt, its image at m + j*ni.  The adjoint
+ * kernels of LinearOperator.rmatmat's column-by-column rmatvecs
+ * (LO:449-460) with the tables, sample indices, weights and window
+ * positions of a wave shared by 4 or 2 models; bitwise each column's
+ * kx_gather. */
+int kx_gather_models(const kx_plan *p, const void *traces, void *m, int32_t nmodel,
+                     void *stream);
+
 /* "same"-mode convolution of each of ntr rows of length nt with an odd
  * filter h[nh]:  y[n] = sum_k h[k] x[n + (nh-1)/2 - k].  x != y. */
 int kx_conv(const void *x, void *y, int64_t ntr, int32_t nt, const void *h,

@@ -27,6 +27,7 @@ EXPORTS = (
     "kx_batch_workspace_bytes", "kx_batch_forward", "kx_batch_adjoint",
     "kx_seg_workspace_bytes", "kx_seg_vec", "kx_seg_dot", "kx_cgls_scalar_batch",
     "kx_eikonal", "kx_conv_rows", "kx_xcorr_rows", "kx_spread_models",
+    "kx_gather_models",
 )
 
 # solver scalar layouts and vector ops (include/kirchx.h)
@@ -187,6 +188,7 @@ def _declare(L):
         "kx_spread": ([PP, P, P, P], ctypes.c_int),
         "kx_gather": ([PP, P, P, P], ctypes.c_int),
         "kx_spread_models": ([PP, P, P, I32, P], ctypes.c_int),
+        "kx_gather_models": ([PP, P, P, I32, P], ctypes.c_int),
         "kx_conv": ([P, P, I64, I32, P, I32, I32, P], ctypes.c_int),
         "kx_conv_rows": ([P, I64, P, I64, I32, P, I64, I32, I32, P], ctypes.c_int),
         "kx_xcorr_rows": ([P, I64, I32, P, I64, I32, P, I64, I32, I32, I32, P], ctypes.c_int),

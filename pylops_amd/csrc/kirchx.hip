@@ -115,11 +115,15 @@ __global__ __launch_bounds__(kSpreadBlock) void spread_models_kernel(
         spread_models_body<TT, DYN, NEG, false, K>(a, m, ni, y, ystride, sc, L % a.ns, L / a.ns);
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+// K models (K > 1: model k's traces at d + k*dstride, its image at
+// m + k*mstride)
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int K>
 __global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD *__restrict__ d,
-                                                              TD *__restrict__ m)
+                                                              TD *__restrict__ m,
+                                                              int64_t dstride, int64_t mstride)
 {
-    gather_tiles<TD, TT, DYN, NEG, JW, P>(a, d, m, xcd_remap(blockIdx.x, gridDim.x));
+    gather_tiles<TD, TT, DYN, NEG, JW, P, K>(a, d, m, xcd_remap(blockIdx.x, gridDim.x), dstride,
+                                             mstride);
 }
 
 template <typename TD>
@@ -442,8 +446,9 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
                : launch_spread_t<TD, TT, false, false>(a, order, p->rblock, max_rb, m, y, sc, st);
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int P>
-int launch_gather_p(const KArgs &a, const void *d, void *m, hipStream_t st)
+template <typename TD, typename TT, bool DYN, bool NEG, int P, int K>
+int launch_gather_p(const KArgs &a, const void *d, int64_t dstride, void *m, int64_t mstride,
+                    hipStream_t st)
 {
     // sources per wave and pass (measured): every pass re-reads the receiver
     // table -- three tables when dynamic -- and more sources hold more
@@ -453,8 +458,9 @@ int launch_gather_p(const KArgs &a, const void *d, void *m, hipStream_t st)
     if ((int64_t)JW * P * a.nr * a.nt >= ((int64_t)1 << 31))
         return kx_fail(KX_ESHAPE, "gather: %d sources x nr x nt exceeds 2^31 samples", JW * P);
     const int nblk = (a.ntiles + NT - 1) / NT;
-    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P>), dim3(nblk), dim3(kGatherBlock),
-                       0, st, a, reinterpret_cast<const TD *>(d), reinterpret_cast<TD *>(m));
+    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P, K>), dim3(nblk),
+                       dim3(kGatherBlock), 0, st, a, reinterpret_cast<const TD *>(d),
+                       reinterpret_cast<TD *>(m), dstride, mstride);
     KX_LAUNCHED("gather");
     return KX_OK;
 }
@@ -462,8 +468,7 @@ int launch_gather_p(const KArgs &a, const void *d, void *m, hipStream_t st)
 // Waves per tile: the fewest that give every CU kGatherFill waves.
 constexpr int kGatherFill = 24;
 
-template <typename TD, typename TT, bool DYN, bool NEG>
-int launch_gather_t(const KArgs &a, int gsplit, const void *d, void *m, hipStream_t st)
+int gather_split(const KArgs &a, int gsplit)
 {
     int P = 1;
     if (gsplit > 0) {
@@ -477,13 +482,20 @@ int launch_gather_t(const KArgs &a, int gsplit, const void *d, void *m, hipStrea
         while (P < kGatherWaves && (int64_t)a.ntiles * P < (int64_t)kGatherFill * cus)
             P *= 2;
     }
-    switch (P) {
+    return P;
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, int K = 1>
+int launch_gather_t(const KArgs &a, int gsplit, const void *d, void *m, hipStream_t st,
+                    int64_t dstride = 0, int64_t mstride = 0)
+{
+    switch (gather_split(a, gsplit)) {
     case 1:
-        return launch_gather_p<TD, TT, DYN, NEG, 1>(a, d, m, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 1, K>(a, d, dstride, m, mstride, st);
     case 2:
-        return launch_gather_p<TD, TT, DYN, NEG, 2>(a, d, m, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 2, K>(a, d, dstride, m, mstride, st);
     default:
-        return launch_gather_p<TD, TT, DYN, NEG, 4>(a, d, m, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 4, K>(a, d, dstride, m, mstride, st);
     }
 }
 
@@ -496,6 +508,33 @@ int launch_gather_d(const kx_plan *p, const KArgs &a, const void *d, void *m, hi
                            : launch_gather_t<TD, TT, true, false>(a, g, d, m, st);
     return p->neg_trav ? launch_gather_t<TD, TT, false, true>(a, g, d, m, st)
                        : launch_gather_t<TD, TT, false, false>(a, g, d, m, st);
+}
+
+// K float32 models per launch, K in {4, 2}, then one by one.
+template <typename TT, bool DYN, bool NEG>
+int launch_gather_models(const kx_plan *p, const KArgs &a, const float *d, float *m, int nmodel,
+                         hipStream_t st)
+{
+    const int64_t ni = (int64_t)p->nx * p->ny * p->nz;
+    const int64_t dstride = (int64_t)p->ns * p->nr * p->nt;
+    const int g = p->gsplit;
+    int j = 0, rc = KX_OK;
+    while (j < nmodel && rc == KX_OK) {
+        const int left = nmodel - j;
+        const float *dj = d + j * dstride;
+        float *mj = m + j * ni;
+        if (left >= 4) {
+            rc = launch_gather_t<float, TT, DYN, NEG, 4>(a, g, dj, mj, st, dstride, ni);
+            j += 4;
+        } else if (left >= 2) {
+            rc = launch_gather_t<float, TT, DYN, NEG, 2>(a, g, dj, mj, st, dstride, ni);
+            j += 2;
+        } else {
+            rc = launch_gather_t<float, TT, DYN, NEG, 1>(a, g, dj, mj, st);
+            j += 1;
+        }
+    }
+    return rc;
 }
 
 }  // namespace
@@ -693,6 +732,33 @@ extern "C" int kx_gather(const kx_plan *p, const void *traces, void *m, void *st
     if (p->tab_fmt == KX_F16)
         return launch_gather_d<float, __half>(p, a, traces, m, st);
     return launch_gather_d<float, float>(p, a, traces, m, st);
+}
+
+extern "C" int kx_gather_models(const kx_plan *p, const void *traces, void *m, int32_t nmodel,
+                                void *stream)
+{
+    int rc = check_plan(p);
+    if (rc)
+        return rc;
+    if (!m || !traces)
+        return kx_fail(KX_EINVAL, "gather_models: NULL buffer");
+    if (p->dtype != KX_F32)
+        return kx_fail(KX_ETYPE, "gather_models: float32 operators only");
+    if (nmodel < 1)
+        return kx_fail(KX_ESHAPE, "gather_models: nmodel=%d", nmodel);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const KArgs a = make_args(p);
+    const float *df = reinterpret_cast<const float *>(traces);
+    float *mf = reinterpret_cast<float *>(m);
+    if (p->tab_fmt == KX_F16)
+        return p->dynamic ? (p->neg_trav ? launch_gather_models<__half, true, true>(p, a, df, mf, nmodel, st)
+                                         : launch_gather_models<__half, true, false>(p, a, df, mf, nmodel, st))
+                          : (p->neg_trav ? launch_gather_models<__half, false, true>(p, a, df, mf, nmodel, st)
+                                         : launch_gather_models<__half, false, false>(p, a, df, mf, nmodel, st));
+    return p->dynamic ? (p->neg_trav ? launch_gather_models<float, true, true>(p, a, df, mf, nmodel, st)
+                                     : launch_gather_models<float, true, false>(p, a, df, mf, nmodel, st))
+                      : (p->neg_trav ? launch_gather_models<float, false, true>(p, a, df, mf, nmodel, st)
+                                     : launch_gather_models<float, false, false>(p, a, df, mf, nmodel, st));
 }
 
 extern "C" int kx_conv_rows(const void *x, int64_t xstride, void *y, int64_t ntr, int32_t nt,

@@ -693,14 +693,20 @@ constexpr int kGatherWaves = 4;
 constexpr int kGatherBlock = kGatherWaves * kWave;
 constexpr int kWin = kWave;  // samples per staged window, one per lane
 
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+// K > 1: the data of K models (model k at d + k*dstride, its image at
+// m + k*mstride) share the tables, sample indices, weights and window
+// positions; each model has its own staged windows and sum, with the
+// single-model operations in the single-model order, so each image is
+// bitwise the one K = 1 gives.
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int K = 1>
 __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restrict__ d,
-                                             TD *__restrict__ m, int L)
+                                             TD *__restrict__ m, int L, int64_t dstride = 0,
+                                             int64_t mstride = 0)
 {
     using TC = typename TimeType<TD>::type;
     constexpr int NT = kGatherWaves / P;  // tiles per workgroup
     constexpr int SB = JW * P;            // sources per block (all parts)
-    __shared__ TD win[kGatherWaves][JW][kWin];
+    __shared__ TD win[kGatherWaves][K][JW][kWin];
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -717,9 +723,12 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
     const TD *wqr = reinterpret_cast<const TD *>(a.wq_r);
     const unsigned ntm1 = (unsigned)(a.nt - 1);
     const int vz = vgpr_zero();
-    TD(&w)[JW][kWin] = win[wave];
+    TD(&w)[K][JW][kWin] = win[wave];
 
-    TD acc = TD(0);
+    TD acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        acc[k] = TD(0);
     if (active) {
         for (int s0 = 0; s0 < a.ns; s0 += SB) {
             // this wave's sources j = s0 + part + P i
@@ -756,7 +765,7 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
             // the sample lies inside the trace; outside ones become zero when
             // the round begins, so that the loads are waited for only then);
             // the aperture gate of the round is loaded with it
-            TD sv[JW];
+            TD sv[K][JW];
             bool svok[JW];
             int izn[JW] = {};
 #define KX_STAGE(R, BR)                                                                        \
@@ -764,7 +773,8 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
     {                                                                                          \
         const int e = bs[i] + (BR) - kWin / 2 + lane;                                          \
         const int ec = min(max(e, 0), a.nt - 1);                                               \
-        sv[i] = dblk[(unsigned)(soff[i] + (R) * a.nt + ec)];                                   \
+        _Pragma("unroll") for (int k = 0; k < K; ++k)                                         \
+            sv[k][i] = dblk[k * dstride + (unsigned)(soff[i] + (R) * a.nt + ec)];              \
         svok[i] = (unsigned)e < (unsigned)a.nt;                                                \
         if constexpr (DYN)                                                                     \
             izn[i] = a.izmin[(unsigned)(izoff[i] + (R))];                                      \
@@ -786,7 +796,9 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                 int izr[JW];
 #pragma unroll
                 for (int i = 0; i < JW; ++i) {
-                    w[i][lane] = svok[i] ? sv[i] : TD(0);
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        w[k][i][lane] = svok[i] ? sv[k][i] : TD(0);
                     izr[i] = izn[i];
                 }
                 const TC orr = (TC)ld_off(&orr_c);
@@ -827,18 +839,23 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                     }
                     const int kk = it - (bs[i] + br - kWin / 2);
                     const int kc = min(max(kk, 0), kWin - 2);
-                    const TD q0 = w[i][kc];
-                    const TD q1 = w[i][kc + 1];
                     const bool in_trace = (unsigned)it < ntm1;
                     miss |= (in_trace && kk != kc) ? (1u << i) : 0u;
-                    TD v = fma((TD)f, q1 - q0, q0);
+                    TD wgt = TD(1);
                     if constexpr (DYN) {
-                        TD wgt = ps[i] * pr + qs[i] * qr;
+                        wgt = ps[i] * pr + qs[i] * qr;
                         if (p.iz < izr[i])
                             wgt = TD(0);
-                        v *= wgt;
                     }
-                    acc += in_trace ? v : TD(0);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const TD q0 = w[k][i][kc];
+                        const TD q1 = w[k][i][kc + 1];
+                        TD v = fma((TD)f, q1 - q0, q0);
+                        if constexpr (DYN)
+                            v *= wgt;
+                        acc[k] += in_trace ? v : TD(0);
+                    }
                 }
                 if (__ballot(miss != 0) != 0) {
                     // replace the window values of the marked sources by the
@@ -858,17 +875,22 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                             }
                             const int kk = it - (bs[i] + br - kWin / 2);
                             const int kc = min(max(kk, 0), kWin - 2);
-                            const TD *q = dblk + (soff[i] + r * a.nt + it);
-                            const TD g0 = q[0], g1 = q[1];
-                            const TD l0 = w[i][kc], l1 = w[i][kc + 1];
-                            TD dv = fma((TD)f, g1 - g0, g0) - fma((TD)f, l1 - l0, l0);
+                            TD wgt = TD(1);
                             if constexpr (DYN) {
-                                TD wgt = ps[i] * pr + qs[i] * qr;
+                                wgt = ps[i] * pr + qs[i] * qr;
                                 if (p.iz < izr[i])
                                     wgt = TD(0);
-                                dv *= wgt;
                             }
-                            acc += dv;
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                const TD *q = dblk + k * dstride + (soff[i] + r * a.nt + it);
+                                const TD g0 = q[0], g1 = q[1];
+                                const TD l0 = w[k][i][kc], l1 = w[k][i][kc + 1];
+                                TD dv = fma((TD)f, g1 - g0, g0) - fma((TD)f, l1 - l0, l0);
+                                if constexpr (DYN)
+                                    dv *= wgt;
+                                acc[k] += dv;
+                            }
                         }
                     }
                 }
@@ -879,18 +901,27 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
     if constexpr (P > 1) {
         // the partial sums of a tile's points, added in part order
         __syncthreads();
-        TD *red = &win[0][0][0];
-        red[threadIdx.x] = acc;
+        TD *red = &win[0][0][0][0];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            red[k * kGatherBlock + threadIdx.x] = acc[k];
         __syncthreads();
         if (part == 0) {
-            acc = TD(0);
 #pragma unroll
-            for (int q = 0; q < P; ++q)
-                acc += red[(q * NT + wave) * kWave + lane];
+            for (int k = 0; k < K; ++k) {
+                acc[k] = TD(0);
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                    acc[k] += red[k * kGatherBlock + (q * NT + wave) * kWave + lane];
+            }
         }
     }
-    if (part == 0 && active && p.valid)
-        m[point_index(a, p)] = acc;
+    if (part == 0 && active && p.valid) {
+        const int64_t pi = point_index(a, p);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            m[k * mstride + pi] = acc[k];
+    }
     if constexpr (P > 1)
         __syncthreads();  // the partial sums are read before a next item stages
 }

@@ -429,12 +429,14 @@ class Kirchhoff(LinearOperator):
         return self._launch(_lib.load().kx_adjoint, x, self.ni, "kx_adjoint")
 
     # several columns at once: the reference's matmat applies matvec column
-    # by column (LO:436-447), and every column here is bitwise its matvec.
+    # by column (LO:436-460), and every column here is bitwise its matvec.
     # A float32 forward spreads up to 8 models per workgroup
     # (kx_spread_models: the tables of a tile are read and the sample index
     # computed once for all of them), then convolves all their traces in
-    # one launch.  Other products run the columns one after another through
-    # the single-problem kernels, on one stream, into one output buffer.  (A
+    # one launch; a float32 adjoint correlates all traces in one launch and
+    # gathers 4 or 2 models per wave (kx_gather_models).  float64 products
+    # run the columns one after another through the single-problem kernels,
+    # on one stream, into one output buffer.  (A
     # persistent batch of k copies of this operator measured 25% slower on
     # the 601x300 dynamic forward than k launches: one problem's launch
     # already fills the GPU, and its receiver rows stay in L2 longer.)
@@ -445,21 +447,36 @@ class Kirchhoff(LinearOperator):
         nout = self.shape[1] if adjoint else self.shape[0]
         xd = to_device(X, self.dtype, self.device).T.contiguous()
         out = torch.empty((k, nout), dtype=xd.dtype, device=self.device)
-        if not adjoint and xd.dtype == torch.float32:
+        if xd.dtype == torch.float32:
             plan = self.plan
             L = _lib.load()
             st = _lib.stream_ptr(self.device)
-            work = torch.empty((min(k, self._MAX_MODELS), nout), dtype=xd.dtype,
+            ntrace = self.nsnr * self.nt
+            work = torch.empty((min(k, self._MAX_MODELS), ntrace), dtype=xd.dtype,
                                device=self.device)
             for j0 in range(0, k, self._MAX_MODELS):
                 nm = min(self._MAX_MODELS, k - j0)
-                rc = L.kx_spread_models(ctypes.byref(plan), _lib.ptr(xd[j0:j0 + nm]),
-                                        _lib.ptr(work), nm, st)
-                _lib.check(rc, "kx_spread_models")
-                rc = L.kx_conv(_lib.ptr(work), _lib.ptr(out[j0:j0 + nm]), nm * self.nsnr,
-                               self.nt, plan.h, plan.nh, _lib.KX_F32, st)
-                _lib.check(rc, "kx_conv")
-            self.matvec_count += k  # one matvec per column, as the reference's
+                if adjoint:
+                    # correlate every column's traces with the wavelet, then
+                    # gather all models (kx_adjoint's two steps, K at once)
+                    rc = L.kx_conv(_lib.ptr(xd[j0:j0 + nm]), _lib.ptr(work), nm * self.nsnr,
+                                   self.nt, plan.hstar, plan.nh, _lib.KX_F32, st)
+                    _lib.check(rc, "kx_conv")
+                    rc = L.kx_gather_models(ctypes.byref(plan), _lib.ptr(work),
+                                            _lib.ptr(out[j0:j0 + nm]), nm, st)
+                    _lib.check(rc, "kx_gather_models")
+                else:
+                    rc = L.kx_spread_models(ctypes.byref(plan), _lib.ptr(xd[j0:j0 + nm]),
+                                            _lib.ptr(work), nm, st)
+                    _lib.check(rc, "kx_spread_models")
+                    rc = L.kx_conv(_lib.ptr(work), _lib.ptr(out[j0:j0 + nm]), nm * self.nsnr,
+                                   self.nt, plan.h, plan.nh, _lib.KX_F32, st)
+                    _lib.check(rc, "kx_conv")
+            # one product per column, as the reference's matmat counts them
+            if adjoint:
+                self.rmatvec_count += k
+            else:
+                self.matvec_count += k
             return like_input(out.T, X)
         for j in range(k):  # each counts one matvec, as the reference's columns do
             self._into(xd[j], out[j], adjoint)

@@ -156,7 +156,7 @@ def test_matmat_forward_models_bitwise(case, k):
     elif case == "neg_byot":
         ts, tr, _, _, _, _ = pylops_amd.Kirchhoff._traveltime_table(G.z, G.x, G.s2d, G.r2d,
                                                                     G.v0, mode="analytic")
-        kw.update(mode="byot", trav=(ts - 0.01, tr), vel=None)
+        kw.update(mode="byot", trav=(ts - 0.01113, tr - 0.00871), vel=None)
     Op = pylops_amd.Kirchhoff(**kw)
     rng = np.random.default_rng(k)
     X = rng.standard_normal((Op.shape[1], k)).astype(np.float32)
@@ -194,3 +194,43 @@ def test_spread_models_rejects_float64():
     rc = _lib.load().kx_spread_models(ctypes.byref(Op.plan), _lib.ptr(m), _lib.ptr(y), 2,
                                       _lib.stream_ptr(torch.device("cuda")))
     assert rc == _lib.KX_ETYPE
+
+
+@pytest.mark.parametrize("case", ["dyn_f32", "dyn_f16", "kin_f32", "kin3d_f16", "neg_byot"])
+@pytest.mark.parametrize("k", [2, 5, 8])
+def test_rmatmat_adjoint_models_bitwise(case, k):
+    """The float32 rmatmat gathers 4 or 2 models per wave
+    (kx_gather_models): every column is bitwise its rmatvec."""
+    import pylops_amd
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dtype="float32")
+    if case == "dyn_f16":
+        kw.update(dynamic=True, tabledtype="float16")
+    elif case == "dyn_f32":
+        kw.update(dynamic=True)
+    elif case == "kin3d_f16":
+        kw.update(y=G.y, srcs=G.s3d, recs=G.r3d, tabledtype="float16")
+    elif case == "neg_byot":
+        ts, tr, _, _, _, _ = pylops_amd.Kirchhoff._traveltime_table(G.z, G.x, G.s2d, G.r2d,
+                                                                    G.v0, mode="analytic")
+        kw.update(mode="byot", trav=(ts - 0.01113, tr - 0.00871), vel=None)
+    Op = pylops_amd.Kirchhoff(**kw)
+    rng = np.random.default_rng(20 + k)
+    V = rng.standard_normal((Op.shape[0], k)).astype(np.float32)
+    V[:, 0] = 0.0
+    Z = Op.rmatmat(V)
+    for j in range(k):
+        np.testing.assert_array_equal(Z[:, j], Op.rmatvec(V[:, j]))
+
+
+@pytest.mark.parametrize("split", [1, 2, 4])
+def test_rmatmat_adjoint_models_source_splits(split):
+    """Every adjoint source split (waves per tile) keeps the models bitwise
+    equal to their own rmatvec."""
+    Op = _kop("float32")
+    Op.gsplit = split
+    rng = np.random.default_rng(31)
+    V = rng.standard_normal((Op.shape[0], 6)).astype(np.float32)
+    Z = Op.rmatmat(V)
+    for j in range(6):
+        np.testing.assert_array_equal(Z[:, j], Op.rmatvec(V[:, j]))
