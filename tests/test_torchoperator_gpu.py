@@ -234,3 +234,17 @@ def test_rmatmat_adjoint_models_source_splits(split):
     Z = Op.rmatmat(V)
     for j in range(6):
         np.testing.assert_array_equal(Z[:, j], Op.rmatvec(V[:, j]))
+
+
+def test_matmat_rmatmat_many_columns():
+    """More columns than one multi-model call takes (64): the products run
+    in chunks and every column stays bitwise its own product."""
+    Op = _kop("float32")
+    rng = np.random.default_rng(41)
+    X = rng.standard_normal((Op.shape[1], 70)).astype(np.float32)
+    V = rng.standard_normal((Op.shape[0], 70)).astype(np.float32)
+    Y = Op.matmat(X)
+    Z = Op.rmatmat(V)
+    for j in (0, 7, 63, 64, 69):
+        np.testing.assert_array_equal(Y[:, j], Op.matvec(X[:, j]))
+        np.testing.assert_array_equal(Z[:, j], Op.rmatvec(V[:, j]))
