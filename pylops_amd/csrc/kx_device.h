@@ -294,12 +294,57 @@ __device__ __forceinline__ int floor_int(float x)
     return i;
 }
 
-// Accumulates one tile (64 points of a wave, RB receivers) into the LDS
-// traces.  The index and fraction of a float time come from one conversion
+// One contribution of weight w at time T samples past the integer base:
+// w (1 - f) to sample k and w f to sample k + 1 of the LDS trace at lrow,
+// k = floor(T) + base, f = T - floor(T).  Lanes with lrow < 0 (no receiver),
+// with `on` false (aperture), or whose k + 1 falls past the trace add
+// nothing.  The index and fraction of a float time come from one conversion
 // and one fract instruction: fract(T) is T - floor(T) rounded as the
 // subtraction rounds it, except that a result that would round to 1 (T in
-// (-2^-24, 0)) stays just below 1.  Dynamic weights are
-// (m ps) pr + (m qs) qr with the source-side products formed once per tile.
+// (-2^-24, 0)) stays just below 1.
+template <typename TD, typename TC, bool NEG, bool FIX>
+__device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int lrow,
+                                              unsigned ntm1, double *accd,
+                                              unsigned long long *acci)
+{
+    TC f;
+    int k;
+    if constexpr (!NEG && sizeof(TC) == 4) {
+        f = __builtin_amdgcn_fractf(T);
+        k = floor_int(T) + base;
+    } else {
+        const TC fl = floor(T);
+        f = T - fl;
+        k = (int)fl + base;
+        if constexpr (NEG) {
+            // the reference truncates toward zero: a time in (-1, 0)
+            // lands on sample 0 with a negative weight
+            if (k == -1 && f > TC(0)) {
+                k = 0;
+                f -= TC(1);
+            }
+        }
+    }
+    if (on && lrow >= 0 && (unsigned)k < ntm1 && w != TD(0)) {
+        const int it = lrow + k;
+        const TD w1 = w * (TD)f;
+        if constexpr (FIX) {
+            // w (1 - f) in one rounding; fixed point by floor(x + 1/2) in
+            // one conversion
+            const TD w0 = fma(-w, (TD)f, w);
+            atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
+            atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
+        } else {
+            const TD w0 = w * (TD)(TC(1) - f);
+            atomicAdd(&accd[it], (double)w0);
+            atomicAdd(&accd[it + 1], (double)w1);
+        }
+    }
+}
+
+// Accumulates one tile (64 points of a wave, RB receivers) into the LDS
+// traces.  Dynamic weights are (m ps) pr + (m qs) qr with the source-side
+// products formed once per tile.
 template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
 __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, RB> &cur, int iz,
                                                   int e,
@@ -323,47 +368,41 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, 
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         const TC T = os + (TC)ld_off(&cur.orr[r]);
-        const int br = step_base<ROT, RB>(cur.br, r);
-        TC f;
-        int k;
-        if constexpr (!NEG && sizeof(TC) == 4) {
-            f = __builtin_amdgcn_fractf(T);
-            k = floor_int(T) + cur.bs + br;
-        } else {
-            const TC fl = floor(T);
-            f = T - fl;
-            k = (int)fl + cur.bs + br;
-            if constexpr (NEG) {
-                // the reference truncates toward zero: a time in (-1, 0)
-                // lands on sample 0 with a negative weight
-                if (k == -1 && f > TC(0)) {
-                    k = 0;
-                    f -= TC(1);
-                }
-            }
-        }
         TD w = mv;
-        bool on = lrow[r] >= 0 && (unsigned)k < ntm1;
+        bool on = true;
         if constexpr (DYN) {
             w = fma(mqs, cur.qr[r], mps * cur.pr[r]);
-            on = on && iz >= izm[r];
+            on = iz >= izm[r];
         }
-        if (on && w != TD(0)) {
-            const int it = lrow[r] + k;
-            const TD w1 = w * (TD)f;
-            if constexpr (FIX) {
-                // w (1 - f) in one rounding; fixed point by
-                // floor(x + 1/2) in one conversion
-                const TD w0 = fma(-w, (TD)f, w);
-                atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
-                atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
-            } else {
-                const TD w0 = w * (TD)(TC(1) - f);
-                atomicAdd(&accd[it], (double)w0);
-                atomicAdd(&accd[it + 1], (double)w1);
-            }
-        }
+        spread_sample<TD, TC, NEG, FIX>(T, cur.bs + step_base<ROT, RB>(cur.br, r), w, on,
+                                        lrow[r], ntm1, accd, acci);
     }
+}
+
+// Zeroes the ntr samples of a workgroup's LDS traces (0.0 and integer 0
+// share the bit pattern).
+__device__ __forceinline__ void traces_clear(unsigned long long *acci, int ntr)
+{
+    for (int i = threadIdx.x; i < ntr; i += blockDim.x)
+        acci[i] = 0ull;
+    __syncthreads();
+}
+
+// Writes the workgroup's ntr LDS samples to yo: fixed point scaled back by
+// 2^-e, or float64 sums rounded to TD.
+template <typename TD, bool FIX>
+__device__ __forceinline__ void traces_store(TD *__restrict__ yo, int ntr, int e,
+                                             const double *accd,
+                                             const unsigned long long *acci)
+{
+    __syncthreads();
+    for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
+        if constexpr (FIX)
+            yo[i] = (TD)ldexp((double)(long long)acci[i], -e);
+        else
+            yo[i] = (TD)accd[i];
+    }
+    __syncthreads();  // the traces are read out before a next item clears them
 }
 
 template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
@@ -378,9 +417,7 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     const int nrl = min(RB, a.nr - r0);
     const int ntr = nrl * a.nt;
 
-    for (int i = threadIdx.x; i < ntr; i += blockDim.x)
-        acci[i] = 0ull;  // 0.0 and integer 0 share the bit pattern
-    __syncthreads();
+    traces_clear(acci, ntr);
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -417,15 +454,7 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
         cur = nxt;
         c = cn;
     }
-    __syncthreads();
-    TD *yo = y + ((size_t)s * a.nr + r0) * a.nt;
-    for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
-        if constexpr (FIX)
-            yo[i] = (TD)ldexp((double)(long long)acci[i], -e);
-        else
-            yo[i] = (TD)accd[i];
-    }
-    __syncthreads();  // the traces are read out before a next item clears them
+    traces_store<TD, FIX>(y + ((size_t)s * a.nr + r0) * a.nt, ntr, e, accd, acci);
 }
 
 // One (source, receiver block) item with the launch's accumulation scale.
