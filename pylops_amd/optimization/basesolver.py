@@ -1,6 +1,7 @@
 """Solver template with callback hooks (pylops/optimization/basesolver.py:18-330)."""
 import functools
 import time
+import weakref
 from abc import ABCMeta, abstractmethod
 
 from pylops_amd.optimization.callback import Callbacks  # noqa: F401
@@ -39,23 +40,30 @@ class Solver(metaclass=ABCMeta):
         print("-" * nbar + "\n")
 
     def _registercallbacks(self):
+        # The wrappers hold the solver weakly and call the plain functions of
+        # its class: a solver is then no reference cycle, and its device
+        # buffers go back to the allocator as soon as it goes out of scope
+        # rather than at the next cyclic garbage collection.
+        ref = weakref.ref(self)
+
         def cbdecorator(func, setup=False):
             @functools.wraps(func)
             def wrapper(*args, **kwargs):
-                if self.callbacks:
-                    for cb in self.callbacks:
+                solver = ref()
+                if solver.callbacks:
+                    for cb in solver.callbacks:
                         arg = kwargs.get("x0", None) if setup else args[0]
-                        getattr(cb, f"on_{func.__name__}_begin")(self, arg)
-                ret = func(*args, **kwargs)
-                if self.callbacks:
-                    for cb in self.callbacks[::-1]:
+                        getattr(cb, f"on_{func.__name__}_begin")(solver, arg)
+                ret = func(solver, *args, **kwargs)
+                if solver.callbacks:
+                    for cb in solver.callbacks[::-1]:
                         arg = kwargs.get("x0", None) if setup else args[0]
-                        getattr(cb, f"on_{func.__name__}_end")(self, arg)
+                        getattr(cb, f"on_{func.__name__}_end")(solver, arg)
                 return ret
             return wrapper
 
         for method in ("setup", "step", "run"):
-            setattr(self, method, cbdecorator(getattr(self, method), method == "setup"))
+            setattr(self, method, cbdecorator(getattr(type(self), method), method == "setup"))
 
     def _wants_host_each_step(self):
         """True when something outside the device needs every iteration:

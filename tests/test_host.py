@@ -266,3 +266,84 @@ def test_convolve1d_construction_rules():
         Convolve1D((3, 50), np.ones((2, 5)), axis=1)  # 2 filters for 3 traces
     C = Convolve1D((3, 50), np.ones((1, 5)), axis=1)
     assert C.h.shape == (3, 9)  # 5 taps padded for offset 0
+
+
+def test_solvers_are_freed_without_cyclic_gc():
+    """A solver holds no reference cycle (its callback wrappers reach it
+    weakly), so it and its device buffers are released as soon as it goes
+    out of scope, before a next solve allocates its own."""
+    import gc
+    import types
+    import weakref
+    from pylops_amd.optimization.cls_basic import CGLS, LSQR
+    from pylops_amd.optimization.cls_sparsity import FISTA, ISTA
+    Op = types.SimpleNamespace(shape=(3, 3))
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        for cls in (CGLS, LSQR, ISTA, FISTA):
+            s = cls(Op)
+            r = weakref.ref(s)
+            del s
+            assert r() is None, cls.__name__
+    finally:
+        if enabled:
+            gc.enable()
+
+
+def test_solver_callback_hooks_wrap_methods():
+    """The on_setup/step/run_begin/end hooks (basesolver.py:18-330) see the
+    solver itself and the method's vector, begin hooks in list order and end
+    hooks in reverse order."""
+    import types
+    from pylops_amd.optimization.basesolver import Solver
+    from pylops_amd.optimization.callback import Callbacks
+
+    class Counting(Solver):
+        def setup(self, y, x0=None, show=False):
+            return 0 if x0 is None else x0
+
+        def step(self, x, show=False):
+            self.iiter += 1
+            return x + 1
+
+        def run(self, x, niter=2, show=False):
+            for _ in range(niter):
+                x = self.step(x)
+            return x
+
+        def finalize(self, show=False):
+            pass
+
+        def memory_usage(self, show=False, unit="B"):
+            return 0
+
+        def solve(self, y, x0=None, niter=2, show=False):
+            x = self.setup(y, x0=x0)
+            x = self.run(x, niter)
+            self.finalize()
+            return x
+
+    seen = []
+
+    class Rec(Callbacks):
+        def __init__(self, tag):
+            self.tag = tag
+
+        def on_setup_begin(self, solver, x0):
+            seen.append((self.tag, "setup_begin", solver, x0))
+
+        def on_step_end(self, solver, x):
+            seen.append((self.tag, "step_end", solver, x))
+
+        def on_run_end(self, solver, x):
+            seen.append((self.tag, "run_end", solver, x))
+
+    s = Counting(types.SimpleNamespace(shape=(1, 1)), callbacks=[Rec("a"), Rec("b")])
+    assert s.solve(None, x0=5, niter=2) == 7
+    assert [(t, e, x) for t, e, _, x in seen] == [
+        ("a", "setup_begin", 5), ("b", "setup_begin", 5),
+        ("b", "step_end", 5), ("a", "step_end", 5),
+        ("b", "step_end", 6), ("a", "step_end", 6),
+        ("b", "run_end", 5), ("a", "run_end", 5)]
+    assert all(sv is s for _, _, sv, _ in seen)
