@@ -373,10 +373,12 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def time_solvers(Op, dobs, niter=30):
-    """Wall time of one CGLS and one LSQR solve of `niter` iterations on the
+def time_solvers(Op, dobs, niter=30, runs=3):
+    """Wall time of a CGLS and an LSQR solve of `niter` iterations on the
     observed data, inputs resident on the device, iterations replayed from
-    a HIP graph (tol = atol = btol = 0, so every iteration runs)."""
+    a HIP graph (tol = atol = btol = 0, so every iteration runs): the median
+    of `runs` solves, each run listed (single solves of one process vary by
+    up to ~20%)."""
     import torch
     from pylops_amd.optimization.basic import cgls, lsqr
     res = {"niter": niter}
@@ -384,11 +386,14 @@ def time_solvers(Op, dobs, niter=30):
                          ("lsqr", lsqr, dict(atol=0.0, btol=0.0, calc_var=False))):
         fn(Op, dobs, niter=3, graph=True, **kw)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = fn(Op, dobs, niter=niter, graph=True, **kw)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        res[f"{name}_s_per_solve"] = round(dt, 4)
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            r = fn(Op, dobs, niter=niter, graph=True, **kw)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        res[f"{name}_s_per_solve"] = round(float(np.median(times)), 4)
+        res[f"{name}_s_runs"] = [round(t, 4) for t in times]
         res[f"{name}_iters"] = int(r[2])
     return res
 

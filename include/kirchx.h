@@ -54,6 +54,14 @@ extern "C" {
 #define KX_ORDER_NATURAL 1
 #define KX_ORDER_ROTATED 2
 
+/* what a forward workgroup blocks (kx_plan.sside): one source and a block of
+ * rblock receivers, or one receiver and a block of rblock sources (float32
+ * operators; float64 ones block receivers).  Both give bitwise the same
+ * traces. */
+#define KX_SIDE_AUTO 0
+#define KX_SIDE_RECEIVERS 1
+#define KX_SIDE_SOURCES 2
+
 /* element types */
 #define KX_F32 0
 #define KX_F64 1
@@ -105,7 +113,7 @@ typedef struct kx_plan {
                      share one aux must be ordered on one stream */
     int32_t gsplit; /* waves per tile in the adjoint, 1, 2 or 4 (each sums a
                        share of the sources), 0 = automatic */
-    int32_t pad1;
+    int32_t sside;  /* forward workgroup blocks KX_SIDE_*, 0 = automatic */
 } kx_plan;
 
 int kx_version(void);

@@ -86,14 +86,14 @@ extern "C" const char *kx_last_error(void) { return g_err; }
 // ---------------------------------------------------------------------------
 namespace {
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB, bool XS>
 __global__ __launch_bounds__(kSpreadBlock) void spread_kernel(KArgs a, const TD *__restrict__ m,
                                                               TD *__restrict__ y,
                                                               const SpreadScale *__restrict__ sc)
 {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const SpreadScale scale = *sc;
-    spread_item<TD, TT, DYN, NEG, ROT, RB>(a, m, y, scale, L % a.ns, (L / a.ns) * RB);
+    spread_item<TD, TT, DYN, NEG, ROT, RB, XS>(a, m, y, scale, L % a.ns, (L / a.ns) * RB);
 }
 
 // K models of a float32 operator; workgroup L (after the XCD renumbering) =
@@ -304,33 +304,34 @@ constexpr int kMaxWavesPerCu = 32;
 
 size_t spread_lds_bytes(int rb, int nt) { return (size_t)rb * nt * 8; }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT, int RB>
 int launch_spread_rb(const KArgs &a, const void *m, void *y, const SpreadScale *sc,
                      hipStream_t st)
 {
     const size_t lds = spread_lds_bytes(RB, a.nt);
     const int nrb = (a.nr + RB - 1) / RB;
     const int nblk = a.ns * nrb;
-    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, ROT, RB>), dim3(nblk), dim3(kSpreadBlock),
-                       lds, st, a, reinterpret_cast<const TD *>(m), reinterpret_cast<TD *>(y), sc);
+    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, ROT, RB, XS>), dim3(nblk),
+                       dim3(kSpreadBlock), lds, st, a, reinterpret_cast<const TD *>(m),
+                       reinterpret_cast<TD *>(y), sc);
     KX_LAUNCHED("spread");
     return KX_OK;
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT>
 const void *spread_fn(int rb)
 {
     switch (rb) {
     case 16:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 16>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 16, XS>);
     case 8:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 8>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 8, XS>);
     case 4:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 4>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 4, XS>);
     case 2:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 2>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 2, XS>);
     default:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, false, 1>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, false, 1, XS>);
     }
 }
 
@@ -353,7 +354,7 @@ int spread_per_cu(const void *fn, int rb, int nt)
 // traces resident per CU, weighted by how far the resident waves (up to 24)
 // can hide the latency of the table loads; ties go to the larger block,
 // which re-reads the image and the source table less often.
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT>
 int pick_rblock(int requested, int nr, int nt, int max_rb)
 {
     auto fits = [&](int rb) {
@@ -370,7 +371,7 @@ int pick_rblock(int requested, int nr, int nt, int max_rb)
     for (int rb : kRblocks) {
         if (!fits(rb) || (rb > 1 && rb / 2 >= nr))
             continue;
-        const int per_cu = spread_per_cu(spread_fn<TD, TT, DYN, NEG, ROT>(rb), rb, nt);
+        const int per_cu = spread_per_cu(spread_fn<TD, TT, DYN, NEG, XS, ROT>(rb), rb, nt);
         const double waves = std::min(24.0, (double)per_cu * kSpreadWaves);
         const double score = waves / 24.0 * per_cu * std::min(rb, nr);
         if (score > best_score * 1.0001) {
@@ -381,32 +382,32 @@ int pick_rblock(int requested, int nr, int nt, int max_rb)
     return best;
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT>
 int launch_spread_o(KArgs a, int requested, int max_rb, const void *m, void *y,
                     const SpreadScale *sc, hipStream_t st)
 {
-    a.rblock = pick_rblock<TD, TT, DYN, NEG, ROT>(requested, a.nr, a.nt, max_rb);
+    a.rblock = pick_rblock<TD, TT, DYN, NEG, XS, ROT>(requested, a.nr, a.nt, max_rb);
     switch (a.rblock) {
     case 16:
-        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 16>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 16>(a, m, y, sc, st);
     case 8:
-        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 8>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 8>(a, m, y, sc, st);
     case 4:
-        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 4>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 4>(a, m, y, sc, st);
     case 2:
-        return launch_spread_rb<TD, TT, DYN, NEG, ROT, 2>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 2>(a, m, y, sc, st);
     default:
-        return launch_spread_rb<TD, TT, DYN, NEG, false, 1>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, false, 1>(a, m, y, sc, st);
     }
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS>
 int launch_spread_t(const KArgs &a, int order, int requested, int max_rb, const void *m, void *y,
                     const SpreadScale *sc, hipStream_t st)
 {
     return order == KX_ORDER_ROTATED
-               ? launch_spread_o<TD, TT, DYN, NEG, true>(a, requested, max_rb, m, y, sc, st)
-               : launch_spread_o<TD, TT, DYN, NEG, false>(a, requested, max_rb, m, y, sc, st);
+               ? launch_spread_o<TD, TT, DYN, NEG, XS, true>(a, requested, max_rb, m, y, sc, st)
+               : launch_spread_o<TD, TT, DYN, NEG, XS, false>(a, requested, max_rb, m, y, sc, st);
 }
 
 
@@ -431,19 +432,51 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
         hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp, sc);
         KX_LAUNCHED("spread scale");
     }
+    // A workgroup blocking sources (float32 operators) runs the spread with
+    // the two sides' tables exchanged and writes its traces to the
+    // transposed positions: T = os + or and the integer bases add in either
+    // order, and the dynamic kernel (XS) forms the source-side products per
+    // step, so the traces are bitwise those of blocks of receivers.
+    // Automatic: a 2D kinematic plan blocks the side with fewer members,
+    // whose rows then serve many workgroups through L2 (13% faster on a
+    // 61 x 301 geometry, either way round); 3D and dynamic plans block
+    // receivers (measured no faster otherwise).
+    int side = p->sside;
+    if (side == KX_SIDE_AUTO)
+        side = (!p->dynamic && p->ny == 1 && p->ns < p->nr) ? KX_SIDE_SOURCES : KX_SIDE_RECEIVERS;
+    const bool xs = sizeof(TD) == 4 && side == KX_SIDE_SOURCES;
+    if (xs) {
+        std::swap(a.ns, a.nr);
+        std::swap(a.off_s, a.off_r);
+        std::swap(a.base_s, a.base_r);
+        std::swap(a.wp_s, a.wp_r);
+        std::swap(a.wq_s, a.wq_r);
+        a.trs = 1;
+        a.trb = p->nr;
+    }
     // automatic lane order: rotated when a step loads one receiver table,
     // natural when it loads three (dynamic weights), where the scattered
     // loads cost more than the collisions they avoid
     int order = p->sorder;
     if (order == KX_ORDER_AUTO)
         order = p->dynamic ? KX_ORDER_NATURAL : KX_ORDER_ROTATED;
+    // exchanged-side dynamic kernels exist for float32 operators only (xs
+    // is false otherwise)
+    constexpr bool kXS = sizeof(TD) == 4;
+    if (p->dynamic && xs)
+        return p->neg_trav ? launch_spread_t<TD, TT, true, true, kXS>(a, order, p->rblock, max_rb,
+                                                                      m, y, sc, st)
+                           : launch_spread_t<TD, TT, true, false, kXS>(a, order, p->rblock,
+                                                                       max_rb, m, y, sc, st);
     if (p->dynamic)
-        return p->neg_trav
-                   ? launch_spread_t<TD, TT, true, true>(a, order, p->rblock, max_rb, m, y, sc, st)
-                   : launch_spread_t<TD, TT, true, false>(a, order, p->rblock, max_rb, m, y, sc, st);
-    return p->neg_trav
-               ? launch_spread_t<TD, TT, false, true>(a, order, p->rblock, max_rb, m, y, sc, st)
-               : launch_spread_t<TD, TT, false, false>(a, order, p->rblock, max_rb, m, y, sc, st);
+        return p->neg_trav ? launch_spread_t<TD, TT, true, true, false>(a, order, p->rblock,
+                                                                        max_rb, m, y, sc, st)
+                           : launch_spread_t<TD, TT, true, false, false>(a, order, p->rblock,
+                                                                         max_rb, m, y, sc, st);
+    return p->neg_trav ? launch_spread_t<TD, TT, false, true, false>(a, order, p->rblock, max_rb,
+                                                                     m, y, sc, st)
+                       : launch_spread_t<TD, TT, false, false, false>(a, order, p->rblock,
+                                                                      max_rb, m, y, sc, st);
 }
 
 template <typename TD, typename TT, bool DYN, bool NEG, int P, int K>

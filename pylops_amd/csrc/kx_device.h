@@ -55,6 +55,10 @@ struct KArgs {
     const int *base_r;
     const void *wp_s, *wq_s, *wp_r, *wq_r;
     const int *izmin;
+    // forward output: the trace of (source s, block member j of a block
+    // starting at r0) is trace s * trs + (r0 + j) * trb; (nr, 1) unless the
+    // forward blocks sources (the side tables exchanged, then (1, ns))
+    int trs, trb;
 };
 
 inline KArgs make_args(const kx_plan *p)
@@ -83,6 +87,8 @@ inline KArgs make_args(const kx_plan *p)
     a.wp_r = p->wp_r;
     a.wq_r = p->wq_r;
     a.izmin = p->izmin;
+    a.trs = p->nr;
+    a.trb = 1;
     return a;
 }
 
@@ -344,8 +350,11 @@ __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int
 
 // Accumulates one tile (64 points of a wave, RB receivers) into the LDS
 // traces.  Dynamic weights are (m ps) pr + (m qs) qr with the source-side
-// products formed once per tile.
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
+// products formed once per tile; with the sides exchanged (XS: the block
+// members are sources) the same products are formed per step, so that the
+// weights round as they do with blocks of receivers.
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB,
+          bool XS = false>
 __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, RB> &cur, int iz,
                                                   int e,
                                                   const int (&lrow)[RB], const int (&izm)[RB],
@@ -361,7 +370,7 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, 
     if constexpr (FIX)
         mv = ldexpf(mv, e);
     TD mps = mv, mqs = TD(0);
-    if constexpr (DYN) {
+    if constexpr (DYN && !XS) {
         mps = mv * cur.ps;
         mqs = mv * cur.qs;
     }
@@ -371,7 +380,10 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, 
         TD w = mv;
         bool on = true;
         if constexpr (DYN) {
-            w = fma(mqs, cur.qr[r], mps * cur.pr[r]);
+            if constexpr (XS)  // (ps, qs) here are the receiver's, pr[r], qr[r] the source's
+                w = fma(mv * cur.qr[r], cur.qs, (mv * cur.pr[r]) * cur.ps);
+            else
+                w = fma(mqs, cur.qr[r], mps * cur.pr[r]);
             on = iz >= izm[r];
         }
         spread_sample<TD, TC, NEG, FIX>(T, cur.bs + step_base<ROT, RB>(cur.br, r), w, on,
@@ -388,24 +400,34 @@ __device__ __forceinline__ void traces_clear(unsigned long long *acci, int ntr)
     __syncthreads();
 }
 
-// Writes the workgroup's ntr LDS samples to yo: fixed point scaled back by
-// 2^-e, or float64 sums rounded to TD.
+// Writes the workgroup's LDS traces (ntr = rows * nt samples) to y: row j
+// to trace first + j * trb; fixed point scaled back by 2^-e, or float64
+// sums rounded to TD.
 template <typename TD, bool FIX>
-__device__ __forceinline__ void traces_store(TD *__restrict__ yo, int ntr, int e,
-                                             const double *accd,
+__device__ __forceinline__ void traces_store(TD *__restrict__ y, size_t first, int trb, int nt,
+                                             int ntr, int e, const double *accd,
                                              const unsigned long long *acci)
 {
     __syncthreads();
+    TD *yo = y + first * nt;
     for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
+        TD v;
         if constexpr (FIX)
-            yo[i] = (TD)ldexp((double)(long long)acci[i], -e);
+            v = (TD)ldexp((double)(long long)acci[i], -e);
         else
-            yo[i] = (TD)accd[i];
+            v = (TD)accd[i];
+        if (trb == 1) {
+            yo[i] = v;  // the block's traces are adjacent
+        } else {
+            const int j = i / nt;
+            yo[(size_t)j * trb * nt + (i - j * nt)] = v;
+        }
     }
     __syncthreads();  // the traces are read out before a next item clears them
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB>
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB,
+          bool XS = false>
 __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, int e, int s, int r0)
 {
@@ -431,8 +453,8 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     for (int r = 0; r < RB; ++r) {
         const int j = step_receiver<RB, ROT>(lane, r);
         lrow[r] = j < nrl ? j * a.nt : -1;
-        if constexpr (DYN)
-            izm[r] = a.izmin[s * a.nr + r0 + (j < nrl ? j : 0)];
+        if constexpr (DYN)  // the gate of a pair is indexed like its trace
+            izm[r] = a.izmin[(size_t)s * a.trs + (size_t)(r0 + (j < nrl ? j : 0)) * a.trb];
     }
     const unsigned ntm1 = (unsigned)(a.nt - 1);
 
@@ -449,27 +471,28 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             spread_load<TD, TT, DYN, ROT, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
-        spread_accumulate<TD, TT, DYN, NEG, ROT, FIX, RB>(cur, c.tzi * a.tz + lz, e, lrow, izm,
-                                                          ntm1, accd, acci);
+        spread_accumulate<TD, TT, DYN, NEG, ROT, FIX, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
+                                                              izm, ntm1, accd, acci);
         cur = nxt;
         c = cn;
     }
-    traces_store<TD, FIX>(y + ((size_t)s * a.nr + r0) * a.nt, ntr, e, accd, acci);
+    traces_store<TD, FIX>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, ntr, e, accd,
+                          acci);
 }
 
 // One (source, receiver block) item with the launch's accumulation scale.
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
+template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB, bool XS = false>
 __device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, const SpreadScale &sc, int s,
                                             int r0)
 {
     if constexpr (sizeof(TD) == 4) {
         if (sc.fixed)
-            spread_body<TD, TT, DYN, NEG, ROT, true, RB>(a, m, y, sc.e, s, r0);
+            spread_body<TD, TT, DYN, NEG, ROT, true, RB, XS>(a, m, y, sc.e, s, r0);
         else
-            spread_body<TD, TT, DYN, NEG, ROT, false, RB>(a, m, y, 0, s, r0);
+            spread_body<TD, TT, DYN, NEG, ROT, false, RB, XS>(a, m, y, 0, s, r0);
     } else {
-        spread_body<TD, TT, DYN, NEG, ROT, false, RB>(a, m, y, 0, s, r0);
+        spread_body<TD, TT, DYN, NEG, ROT, false, RB, XS>(a, m, y, 0, s, r0);
     }
 }
 

@@ -83,25 +83,49 @@ def _apply_into(Op, x, out, adjoint):
 
 def _replay(graph, n, sc, active_index, device):
     """Replay a captured iteration n times, looking at the loop flag every
-    few replays so that a converged solve stops early."""
-    for k in range(n):
-        if k % _CHECK_EVERY == 0 and k > 0 and sc[active_index].item() == 0.0:
-            break
-        graph.replay()
-
-
-def _capture(body, device):
-    """Capture ``body`` in a HIP graph.  Python's cyclic garbage collector is
-    kept off during the capture (torch collects once before it begins):
-    collecting an earlier solver there would destroy its graph while the
-    stream is capturing, which HIP forbids."""
-    graph = torch.cuda.CUDAGraph()
-    torch.cuda.synchronize(device)
+    few replays so that a converged solve stops early.  Python's cyclic
+    garbage collector is held off meanwhile: the loop allocates next to
+    nothing, and a full collection (tens of ms in a process with many
+    objects) just after a flag check would leave the device idle."""
     enabled = gc.isenabled()
     gc.disable()
     try:
-        with torch.cuda.graph(graph):
-            body()
+        for k in range(n):
+            if k % _CHECK_EVERY == 0 and k > 0 and sc[active_index].item() == 0.0:
+                break
+            graph.replay()
+    finally:
+        if enabled:
+            gc.enable()
+
+
+_capture_streams = {}
+
+
+def _capture(body, device):
+    """Capture ``body`` in a HIP graph on a side stream (one per device,
+    default priority, reused).  torch's ``graph`` context would first empty
+    the allocator's cache, so that every solve allocated its device vectors
+    afresh; capture_begin / capture_end keep the cache.  Python's cyclic
+    garbage collector is kept off during the capture: collecting an earlier
+    solver there would destroy its graph while the stream is capturing,
+    which HIP forbids."""
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize(device)
+    dev = torch.device(device)
+    key = dev.index if dev.index is not None else torch.cuda.current_device()
+    stream = _capture_streams.get(key)
+    if stream is None:
+        stream = _capture_streams[key] = torch.cuda.Stream(device=device)
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.stream(stream):
+            graph.capture_begin()
+            try:
+                body()
+            finally:
+                graph.capture_end()
     finally:
         if enabled:
             gc.enable()

@@ -5,8 +5,9 @@ change their results beyond rounding (the float32 forward not at all):
 
 - ``tile`` (tz, tx, ty), the image block of one wavefront (the tables are
   rebuilt for each candidate);
-- ``rblock``, receivers per forward workgroup, and ``sorder``, the forward's
-  lane order;
+- ``rblock``, receivers (or sources) per forward workgroup, ``sorder``,
+  the forward's lane order, and ``sside``, which side a forward workgroup
+  blocks (single problems);
 - ``gsplit``, adjoint waves per tile.
 
 :func:`autotune` times every candidate on the device with a dense random
@@ -34,6 +35,7 @@ TILES_2D = ((16, 4, 1), (8, 8, 1), (32, 2, 1), (4, 16, 1))
 TILES_3D = ((16, 2, 2), (8, 4, 2), (4, 4, 4), (8, 8, 1), (16, 4, 1))
 RBLOCKS = (16, 8, 4, 2, 1)
 SORDERS = (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED)
+SSIDES = (_lib.KX_SIDE_RECEIVERS, _lib.KX_SIDE_SOURCES)
 GSPLITS = (1, 2, 4)
 
 
@@ -99,25 +101,29 @@ def _timer(fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def _candidates_fwd(dtype, dynamic, nt):
+def _candidates_fwd(dtype, dynamic, nt, sides=(_lib.KX_SIDE_RECEIVERS,)):
     max_rb = 4 if dtype == "float64" else (8 if dynamic else 16)
+    if dtype == "float64":
+        sides = (_lib.KX_SIDE_RECEIVERS,)  # float64 operators block receivers
     for rb in RBLOCKS:
         if rb > max_rb or (rb > 1 and rb * nt * 8 > 65536):
             continue  # more registers than two workgroups per CU, or LDS
         for so in SORDERS if rb > 1 else (_lib.KX_ORDER_NATURAL,):
-            yield rb, so
+            for side in sides:
+                yield rb, so, side
 
 
 def autotune(reps=3, tiles=None, cache=None, **kw):
     """Fastest launch shapes for ``Kirchhoff(**kw)``: returns a dict with
-    ``tile``, ``rblock``, ``sorder``, ``gsplit`` (pass it on to
+    ``tile``, ``rblock``, ``sorder``, ``sside``, ``gsplit`` (pass it on to
     ``Kirchhoff``) and the measured ``ms`` (forward kernel + adjoint
     kernel).  ``tiles`` overrides the candidate tiles; ``cache`` is the
     JSON cache file (default ``$PYLOPS_AMD_TUNE_CACHE`` or
     ``~/.cache/pylops_amd/tune.json``; ``False`` disables it)."""
     from pylops_amd.waveeqprocessing.kirchhoff import Kirchhoff
     _lib.require_device()
-    kw = {k: v for k, v in kw.items() if k not in ("tile", "rblock", "sorder", "gsplit")}
+    kw = {k: v for k, v in kw.items()
+          if k not in ("tile", "rblock", "sorder", "gsplit", "sside")}
     device = kw.get("device") or torch.device("cuda", torch.cuda.current_device())
     path = None if cache is False else _cache_path(cache)
     key = _key("op", [kw], device)
@@ -153,16 +159,17 @@ def autotune(reps=3, tiles=None, cache=None, **kw):
                        "kx_gather")
 
         fwd = []
-        for rb, so in _candidates_fwd(dtype, kw.get("dynamic", False), len(kw["t"])):
-            plan.rblock, plan.sorder = rb, so
-            fwd.append((_timer(spread, reps), rb, so))
+        for rb, so, side in _candidates_fwd(dtype, kw.get("dynamic", False), len(kw["t"]),
+                                            SSIDES):
+            plan.rblock, plan.sorder, plan.sside = rb, so, side
+            fwd.append((_timer(spread, reps), rb, so, side))
         adj = []
         for gs in GSPLITS:
             plan.gsplit = gs
             adj.append((_timer(gather, reps), gs))
-        tf, rb, so = min(fwd)
+        tf, rb, so, side = min(fwd)
         ta, gs = min(adj)
-        cand = {"tile": tuple(tile), "rblock": rb, "sorder": so, "gsplit": gs,
+        cand = {"tile": tuple(tile), "rblock": rb, "sorder": so, "sside": side, "gsplit": gs,
                 "ms": round(tf + ta, 4), "ms_forward": round(tf, 4), "ms_adjoint": round(ta, 4)}
         if best is None or cand["ms"] < best["ms"]:
             best = cand
@@ -207,7 +214,7 @@ def autotune_batch(kws, reps=3, tiles=None, cache=None):
         a = torch.empty(B0.shape[1], dtype=tdt, device=B0.device)
         B0._into(m, d, False)
         fwd = []
-        for rb, so in _candidates_fwd(dtype, dynamic, max(len(kw["t"]) for kw in kws)):
+        for rb, so, _ in _candidates_fwd(dtype, dynamic, max(len(kw["t"]) for kw in kws)):
             B = KirchhoffBatch(ops, rblock=rb, sorder=so)
             if B.rblock != rb:
                 continue  # does not fit the batch's longest trace

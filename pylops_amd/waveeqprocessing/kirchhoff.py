@@ -32,7 +32,7 @@ _ENGINES = ("hip", "cuda", "numba", "numpy")
 # small attributes whose reassignment invalidates the device tables
 _PLAIN_ATTRS = ("vel", "aperture", "angleaperture", "six", "rix", "tile", "tabledtype")
 # launch-shape attributes: copied into the plan, the tables stay
-_LAUNCH_ATTRS = ("rblock", "sorder", "gsplit")
+_LAUNCH_ATTRS = ("rblock", "sorder", "gsplit", "sside")
 
 
 class _Table:
@@ -75,13 +75,18 @@ class Kirchhoff(LinearOperator):
         (float32 operators only); default is the operator dtype.
     tile : ``(tz, tx, ty)`` image tile of 64 points.
     device : torch device of the tables (default: current HIP device).
-    rblock : receivers per forward workgroup (0 = automatic).
+    rblock : receivers (or sources, see ``sside``) per forward workgroup
+        (0 = automatic).
     sorder : forward lane order, ``KX_ORDER_NATURAL`` (1) or
         ``KX_ORDER_ROTATED`` (2); 0 = automatic.
     gsplit : adjoint waves per tile, 1, 2 or 4; 0 = automatic.
+    sside : what a forward workgroup blocks, ``KX_SIDE_RECEIVERS`` (1: one
+        source, a block of receivers) or ``KX_SIDE_SOURCES`` (2: one
+        receiver, a block of sources; float32 operators); 0 = automatic.
+        Every choice gives bitwise the same data.
 
     ``waveeqprocessing.autotune`` measures the fastest ``tile``, ``rblock``,
-    ``sorder`` and ``gsplit`` for a geometry.
+    ``sorder``, ``sside`` and ``gsplit`` for a geometry.
     """
 
     trav_srcs = _Table()
@@ -96,7 +101,7 @@ class Kirchhoff(LinearOperator):
                  mode="eikonal", wavfilter=False, dynamic=False, trav=None, amp=None,
                  aperture=None, angleaperture=90.0, snell=None, engine="hip",
                  dtype="float64", name="K", *, tabledtype=None, tile=None,
-                 device=None, rblock=0, sorder=0, gsplit=0):
+                 device=None, rblock=0, sorder=0, gsplit=0, sside=0):
         object.__setattr__(self, "_dirty", True)
         object.__setattr__(self, "_tab", {})
         object.__setattr__(self, "_tab_host", {})
@@ -212,6 +217,7 @@ class Kirchhoff(LinearOperator):
         self.tile = tuple(tile) if tile is not None else T.default_tile(self.ndims, self.nz)
         self.rblock = int(rblock)
         self.sorder = int(sorder)
+        self.sside = int(sside)
         self.gsplit = int(gsplit)
         self.engine = engine
         if engine == "cuda" and not self.travsrcrec:
@@ -368,6 +374,7 @@ class Kirchhoff(LinearOperator):
         plan.neg_trav = int(neg)
         plan.rblock = self.rblock
         plan.sorder = self.sorder
+        plan.sside = self.sside
         plan.gsplit = self.gsplit
         plan.nh = keep["h"].numel()
         for k, v in keep.items():

@@ -230,11 +230,12 @@ def test_autotune_single_operator(tmp_path):
     best = autotune(cache=cache, reps=1, **kw)
     assert best["tile"][0] * best["tile"][1] * best["tile"][2] == 64
     assert best["rblock"] in (1, 2, 4, 8) and best["gsplit"] in (1, 2, 4)
+    assert best["sside"] in (1, 2)
     assert best["ms"] > 0
     assert autotune(cache=cache, reps=1, **kw) == best  # from the cache
     op0 = pylops_amd.Kirchhoff(**kw)
     op = pylops_amd.Kirchhoff(**kw, tile=best["tile"], rblock=best["rblock"],
-                              sorder=best["sorder"], gsplit=best["gsplit"])
+                              sorder=best["sorder"], sside=best["sside"], gsplit=best["gsplit"])
     m = rng.standard_normal(op.shape[1]).astype(np.float32)
     d = rng.standard_normal(op.shape[0]).astype(np.float32)
     # another tile splits traveltimes into other integer bases and offsets:

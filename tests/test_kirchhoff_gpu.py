@@ -257,6 +257,65 @@ def test_spread_lane_orders_and_receiver_blocks(dynamic):
         np.testing.assert_array_equal(got, outs[0])
 
 
+@pytest.mark.parametrize("case", ["kin2d", "kin2d_f16", "kin3d", "neg_byot", "nonfinite",
+                                  "dynamic", "dynamic_f16_neg", "dynamic3d"])
+def test_spread_source_blocks_bitwise(case):
+    """Workgroups blocking sources (KX_SIDE_SOURCES: one receiver and a block
+    of sources, the side tables exchanged) give traces bitwise equal to
+    blocks of receivers, for every receiver block and lane order, with more
+    and fewer sources than receivers, float16 tables, 3D and the
+    negative-time path and dynamic weights (their products formed per step,
+    so that they round as with blocks of receivers); a non-finite model
+    keeps its NaN pattern."""
+    import pylops_amd
+    from pylops_amd import _lib
+    srcs = np.vstack((np.linspace(G.x.min(), G.x.max(), 5), 2 * np.ones(5)))
+    recs = np.vstack((np.linspace(G.x.min(), G.x.max(), 7), 2 * np.ones(7)))
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=srcs, recs=recs, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dtype="float32")
+    if case == "kin2d_f16":
+        kw.update(srcs=recs, recs=srcs, tabledtype="float16")
+    elif case == "kin3d":
+        kw.update(y=G.y, srcs=G.s3d_par, recs=G.r3d_par)
+    elif case == "neg_byot":
+        ts, tr, _, _, _, _ = pylops_amd.Kirchhoff._traveltime_table(G.z, G.x, srcs, recs,
+                                                                    G.v0, mode="analytic")
+        kw.update(mode="byot", trav=(ts - 0.01113, tr - 0.00871), vel=None)
+    elif case == "dynamic":
+        kw.update(dynamic=True, angleaperture=60)
+    elif case == "dynamic_f16_neg":
+        ts, tr, ds, dr, _, _ = pylops_amd.Kirchhoff._traveltime_table(G.z, G.x, srcs, recs,
+                                                                      G.v0, mode="analytic")
+        kw.update(mode="byot", trav=(ts - 0.01113, tr - 0.00871),
+                  amp=(1.0 / np.maximum(ds, 1.0), 1.0 / np.maximum(dr, 1.0)),
+                  dynamic=True, tabledtype="float16")
+    elif case == "dynamic3d":
+        kw.update(y=G.y, srcs=G.s3d, recs=G.r3d, dynamic=True, angleaperture=70)
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**{k: v for k, v in kw.items() if k != "tabledtype"})
+    m = np.random.default_rng(21).standard_normal(Op.shape[1]).astype(np.float32)
+    if case == "nonfinite":
+        m[40] = np.nan
+    Op.plan.sside = _lib.KX_SIDE_RECEIVERS
+    Op.plan.sorder = _lib.KX_ORDER_NATURAL
+    Op.plan.rblock = 1
+    want = np.asarray(Op.spread(m))
+    if case not in ("nonfinite", "kin2d_f16", "dynamic_f16_neg"):  # float16: operator tests
+        assert G.relerr(want, orc.spread(m)) < 1e-5
+    for order in (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED):
+        for rb in (1, 2, 4, 8, 16):
+            Op.plan.sside = _lib.KX_SIDE_SOURCES
+            Op.plan.sorder = order
+            Op.plan.rblock = rb
+            got = np.asarray(Op.spread(m))
+            if case == "nonfinite":
+                np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+                ok = ~np.isnan(want)
+                assert G.relerr(got[ok], want[ok]) < 1e-12, (order, rb)
+            else:
+                np.testing.assert_array_equal(got, want, err_msg=f"order={order} rblock={rb}")
+
+
 def test_spread_nonfinite_model_propagates_like_reference():
     """inf / NaN in the model switch the launch to float64 accumulation:
     the samples they reach become non-finite exactly where the oracle's
