@@ -377,14 +377,14 @@ def time_solvers(Op, dobs, niter=30, runs=3):
     """Wall time of a CGLS and an LSQR solve of `niter` iterations on the
     observed data, inputs resident on the device, iterations replayed from
     a HIP graph (tol = atol = btol = 0, so every iteration runs): the median
-    of `runs` solves, each run listed (single solves of one process vary by
-    up to ~20%)."""
+    of `runs` solves, each run listed.  The warm-up solve runs past the
+    replay loop's first flag check, which costs ~40 ms once per process."""
     import torch
     from pylops_amd.optimization.basic import cgls, lsqr
     res = {"niter": niter}
     for name, fn, kw in (("cgls", cgls, dict(tol=0.0)),
                          ("lsqr", lsqr, dict(atol=0.0, btol=0.0, calc_var=False))):
-        fn(Op, dobs, niter=3, graph=True, **kw)
+        fn(Op, dobs, niter=10, graph=True, **kw)  # past the replay loop's first flag check
         torch.cuda.synchronize()
         times = []
         for _ in range(runs):
