@@ -445,15 +445,8 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
     if (side == KX_SIDE_AUTO)
         side = (!p->dynamic && p->ny == 1 && p->ns < p->nr) ? KX_SIDE_SOURCES : KX_SIDE_RECEIVERS;
     const bool xs = sizeof(TD) == 4 && side == KX_SIDE_SOURCES;
-    if (xs) {
-        std::swap(a.ns, a.nr);
-        std::swap(a.off_s, a.off_r);
-        std::swap(a.base_s, a.base_r);
-        std::swap(a.wp_s, a.wp_r);
-        std::swap(a.wq_s, a.wq_r);
-        a.trs = 1;
-        a.trb = p->nr;
-    }
+    if (xs)
+        exchange_sides(a);
     // automatic lane order: rotated when a step loads one receiver table,
     // natural when it loads three (dynamic weights), where the scattered
     // loads cost more than the collisions they avoid

@@ -92,6 +92,30 @@ inline KArgs make_args(const kx_plan *p)
     return a;
 }
 
+// The forward's arguments with the two sides exchanged: its workgroups then
+// hold one receiver and a block of sources, and write each trace to its
+// transposed position (kx_plan.sside).
+inline void exchange_sides(KArgs &a)
+{
+    const int ns = a.ns, nr = a.nr;
+    a.ns = nr;
+    a.nr = ns;
+    const void *o = a.off_s;
+    a.off_s = a.off_r;
+    a.off_r = o;
+    const int *b = a.base_s;
+    a.base_s = a.base_r;
+    a.base_r = b;
+    const void *w = a.wp_s;
+    a.wp_s = a.wp_r;
+    a.wp_r = w;
+    w = a.wq_s;
+    a.wq_s = a.wq_r;
+    a.wq_r = w;
+    a.trs = 1;
+    a.trb = nr;
+}
+
 struct TilePoint {
     int ix, iy, iz;
     bool valid;
