@@ -131,6 +131,7 @@ def autotune(reps=3, tiles=None, cache=None, **kw):
         hit = _load(path).get(key)
         if hit is not None:
             hit["tile"] = tuple(hit["tile"])
+            hit.setdefault("sside", _lib.KX_SIDE_AUTO)  # entries from before sside was tuned
             return hit
     ndims = _geometry(kw)[0]
     tiles = tiles or (TILES_2D if ndims == 2 else TILES_3D)
