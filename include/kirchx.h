@@ -95,7 +95,7 @@ typedef struct kx_plan {
     int32_t ntz, ntx, nty;
     int64_t ntiles;      /* ntz*ntx*nty */
     int32_t neg_trav;    /* 1 if a traveltime may be negative (truncation path) */
-    int32_t rblock;      /* receivers per spread workgroup, 0 = automatic */
+    int32_t rblock;      /* receivers (sources, see sside) per spread workgroup, 0 = automatic */
     int32_t nh;          /* wavelet length after offset padding (odd) */
     int32_t sorder;      /* spread lane order KX_ORDER_*, 0 = automatic */
     const void *off_s;
