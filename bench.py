@@ -377,8 +377,9 @@ def time_solvers(Op, dobs, niter=30, runs=3):
     """Wall time of a CGLS and an LSQR solve of `niter` iterations on the
     observed data, inputs resident on the device, iterations replayed from
     a HIP graph (tol = atol = btol = 0, so every iteration runs): the median
-    of `runs` solves, each run listed.  The warm-up solve runs past the
-    replay loop's first flag check, which costs ~40 ms once per process."""
+    of `runs` solves, each run listed (the first timed CGLS solve of a
+    process is at times ~40 ms slower).  The warm-up solve runs past the
+    replay loop's first flag check."""
     import torch
     from pylops_amd.optimization.basic import cgls, lsqr
     res = {"niter": niter}
