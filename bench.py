@@ -1,7 +1,8 @@
 """Kirchhoff forward + adjoint throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2d|3d|2dsmall|batch|autograd]
-                    [--tables f32|f16] [--no-cpu] [--extra] [--no-solve] [--no-batch]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload 3d|2d|2dsmall|batch|autograd]
+                    [--no-cpu] [--no-solve] [--no-extra] [--calib-copy]
 
 A step is one forward (spread + wavelet convolution) of a dense image and
 one adjoint (wavelet correlation + gather) of the observed data, i.e. the
@@ -10,13 +11,17 @@ migrated image K^T d (what the solver's forward sees after its first
 adjoint), so no tile of the forward is skipped as empty.
 
 Workloads (synthetic inputs, fixed seeds):
+  3d       3D, byot straight-ray traveltimes for v(z) = 1800 + 0.6 z m/s;
+           grid 120 x 128 x 128 at 10 m; nt=1001 at 4 ms; 5x5 sources and
+           20x20 receivers on the surface; kinematic.          (default)
+           The line's value is the float32-table step; the float16
+           offset-table step, CGLS (20 iterations) s/solve for both, their
+           errors against the float64 operator and the CPU baseline sit
+           beside it, and under "extra" the 2d line and the batch.
   2d       2D, analytic constant velocity 2500 m/s, dynamic=True; grid
            nz=300 x nx=601 at 5 m; nt=1501 at 2 ms; 61 sources and 301
            receivers on the surface; 25 Hz Ricker; 8 dipping reflectors and
-           point diffractors.                                   (default)
-  3d       3D, byot straight-ray traveltimes for v(z) = 1800 + 0.6 z m/s;
-           grid 120 x 128 x 128 at 10 m; nt=1001 at 4 ms; 5x5 sources and
-           20x20 receivers on the surface; kinematic.
+           point diffractors; LSQR and CGLS 30 iterations.
   2dsmall  2D, 2000 m/s, kinematic; 100 x 101 at 10 m; nt=651 at 4 ms;
            11 sources, 51 receivers.
   batch    64 independent 2D problems of uneven sizes (nz U[64,256], nx
@@ -159,20 +164,22 @@ def workload_2dsmall(rng):
     return kw, refl, desc
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` (float32 tables) from
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries in profiles/
-    (a 'TRAFFIC <workload> <kernel> <bytes>' line), or None."""
+def measured_traffic(workload, kernel, tables="f32"):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summaries in profiles/ (a
+    'TRAFFIC <workload> <kernel> <bytes> [tables]' line), with the file it
+    came from, or (None, None)."""
     import glob
     import re
     here = os.path.dirname(os.path.abspath(__file__))
     for path in sorted(glob.glob(os.path.join(here, "profiles", "*.txt"))):
         with open(path) as f:
             for line in f:
-                m = re.match(r"TRAFFIC (\S+) (\S+) (\d+)", line)
-                if m and m.group(1) == workload and m.group(2) == kernel:
-                    return int(m.group(3))
-    return None
+                m = re.match(r"TRAFFIC (\S+) (\S+) (\d+)(?: (\S+))?", line)
+                if (m and m.group(1) == workload and m.group(2) == kernel
+                        and m.group(4) in (None, tables)):
+                    return int(m.group(3)), os.path.join("profiles", os.path.basename(path))
+    return None, None
 
 
 def algorithmic_bytes(desc, ni, table_bytes, dtype_bytes=4):
@@ -188,104 +195,121 @@ def algorithmic_bytes(desc, ni, table_bytes, dtype_bytes=4):
     return b
 
 
-def cpu_baseline(kw, desc, m, d, budget_s=12.0):
+def compulsory_bytes(Op, dtype_bytes=4):
+    """Bytes one kernel launch (spread or gather) must move at least: every
+    table element, weight and base once (tile-padded device layout), the
+    image once and the traces once; dynamic plans add the two weight tables
+    per side and the per-trace aperture gate."""
+    plan = Op.plan
+    ntiles = int(plan.ntiles)
+    nsr = Op.ns + Op.nr
+    tb = 2 if Op.tabledtype == np.float16 else (8 if Op.dtype == np.float64 else 4)
+    b = nsr * ntiles * 64 * tb + nsr * ntiles * 4
+    if Op.dynamic:
+        b += 2 * nsr * ntiles * 64 * dtype_bytes + Op.nsnr * 4
+    return b + Op.ni * dtype_bytes + Op.nsnr * Op.nt * dtype_bytes
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(kw, desc, m, d, target_s=2.0, runs=3):
     """The oracle (C/OpenMP restatement of the reference kernels, float32
-    operator arithmetic) on a bounded sample: the full image and receiver
-    set with a subset of the sources, forward + adjoint."""
+    operator arithmetic, the reference's loop nests: forward parallel over
+    sources, adjoint over image points) timed on a bounded sample of the
+    workload -- the full image and receiver set with a subset of the
+    sources -- at all host cores and at one thread: one warm-up run, then
+    the median of `runs` timed forward + adjoint runs."""
     from oracle import oracle as O
     import torch
+    L = O.lib()
     kwo = {k: v for k, v in kw.items()}
     if kwo.get("mode") == "byot":
         kwo["trav"] = tuple(a.cpu().numpy() if isinstance(a, torch.Tensor) else a
                             for a in kwo["trav"])
     kwo["dtype"] = "float32"
     nsrc_total = kwo["srcs"].shape[1]
-    nthreads = O.lib().or_num_threads()
+    nall = int(L.or_num_threads())
+    dd_all = np.asarray(d).reshape(nsrc_total, -1)
+    mm = np.asarray(m, dtype=np.float32).ravel()
 
-    def run(nsrc):
+    def make(nsrc):
         k2 = dict(kwo)
         k2["srcs"] = kwo["srcs"][:, :nsrc]
         if k2.get("mode") == "byot":
-            k2["trav"] = (kwo["trav"][0][:, :nsrc], kwo["trav"][1])
-        orc = O.KirchhoffOracle(**k2)
-        dd = np.asarray(d).reshape(nsrc_total, -1)[:nsrc].ravel()
+            k2["trav"] = (np.ascontiguousarray(kwo["trav"][0][:, :nsrc]), kwo["trav"][1])
+        return O.KirchhoffOracle(**k2), dd_all[:nsrc].ravel()
+
+    def once(orc, dd):
         t0 = time.perf_counter()
-        orc.matvec(np.asarray(m, dtype=np.float32).ravel())
+        orc.matvec(mm)
         orc.rmatvec(dd)
-        return time.perf_counter() - t0, orc
+        return time.perf_counter() - t0
 
-    n1 = min(nthreads, nsrc_total)
-    dt1, orc = run(n1)
-    nsrc = n1
-    if dt1 < budget_s / 2:
-        nsrc = int(min(nsrc_total, max(n1, n1 * budget_s / max(dt1, 1e-3))))
-        nsrc = max(n1, (nsrc // n1) * n1)
-        if nsrc != n1:
-            dt1, orc = run(nsrc)
-    trip = 2.0 * orc.ni * nsrc * orc.nr
-    return dict(value=trip / dt1 / 1e9, unit="Gsamples/s", cores=int(nthreads), kind="port",
-                sample=f"{desc['workload']} with {nsrc} of {nsrc_total} sources, "
-                       f"one forward + one adjoint (float32 operator arithmetic), "
-                       f"{dt1:.1f} s")
+    res = {}
+    try:
+        for label, nthr in (("all_cores", nall), ("one_thread", 1)):
+            L.or_set_num_threads(nthr)
+            # sources of the sample: a multiple of the thread count (the
+            # forward runs one source per thread), sized from a probe run to
+            # about target_s per run
+            n1 = min(nthr, nsrc_total)
+            orc, dd = make(n1)
+            t1 = once(orc, dd)
+            nsrc = n1
+            if t1 < target_s / 2:
+                nsrc = int(min(nsrc_total, n1 * target_s / max(t1, 1e-3)))
+                nsrc = max(n1, (nsrc // n1) * n1)
+                if nsrc != n1:
+                    orc, dd = make(nsrc)
+            once(orc, dd)  # warm-up
+            ts = [once(orc, dd) for _ in range(runs)]
+            t = float(np.median(ts))
+            trip = 2.0 * orc.ni * nsrc * orc.nr
+            res[label] = dict(value=round(trip / t / 1e9, 4), threads=nthr, sources=nsrc,
+                              s_runs=[round(x, 3) for x in ts])
+    finally:
+        L.or_set_num_threads(nall)
+    a = res["all_cores"]
+    return dict(value=a["value"], unit="Gsamples/s", cores=nall, kind="port",
+                cpu=cpu_model(), one_thread=res["one_thread"], all_cores=a,
+                sample=f"{desc['workload']}: all {orc.nr} receivers and the full image "
+                       f"with the first {a['sources']} of {nsrc_total} sources at "
+                       f"{nall} threads ({res['one_thread']['sources']} at one "
+                       f"thread); one forward + one adjoint (float32 operator "
+                       f"arithmetic, float64 tables), median of {runs} after a warm-up")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="2d", choices=["2d", "3d", "2dsmall", "batch", "autograd"])
-    ap.add_argument("--tables", default="f32", choices=["f32", "f16"])
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extra", action="store_true",
-                    help="also the 3D precision report (float32 and float16 tables vs float64, "
-                         "fwd, adj and 20 CGLS iterations)")
-    ap.add_argument("--no-solve", action="store_true",
-                    help="skip the 30-iteration CGLS / LSQR timings and the 2D precision "
-                         "report (both run solvers)")
-    ap.add_argument("--adam-steps", type=int, default=50,
-                    help="Adam steps of the autograd workload")
-    ap.add_argument("--no-batch", action="store_true",
-                    help="skip the batch of 64 uneven 2D problems (batched CGLS)")
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="run one 4 GiB device copy first (reference kernel for "
-                         "FETCH_SIZE / WRITE_SIZE counter scaling)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU sharding is a design note only)"}))
-        sys.exit(2)
+def build_workload(name, dev):
+    rng = np.random.default_rng(1003 if name == "3d" else 1002)
+    if name == "2d":
+        return workload_2d(rng)
+    if name == "3d":
+        return workload_3d(rng, dev)
+    return workload_2dsmall(rng)
 
+
+def time_operator(kw, refl, tables, dev, steps, warmup):
+    """A step = kx_spread of the dense image K^T d + its wavelet convolution
+    + the wavelet correlation of d + kx_gather, each kernel bracketed by
+    HIP events on the launch stream.  Returns the timings, the operator and
+    the observed data."""
+    import ctypes
     import torch
     import pylops_amd
     from pylops_amd import _lib
-
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    if args.calib_copy:
-        src = torch.ones(1024 * 1024 * 1024, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    if args.workload == "batch":
-        print(json.dumps(batch_line(dev, args)), flush=True)
-        return
-    if args.workload == "autograd":
-        print(json.dumps(autograd_line(dev, args, nsteps=args.adam_steps)), flush=True)
-        return
-    rng = np.random.default_rng(1002)
-    if args.workload == "2d":
-        kw, refl, desc = workload_2d(rng)
-    elif args.workload == "3d":
-        kw, refl, desc = workload_3d(rng, dev)
-    else:
-        kw, refl, desc = workload_2dsmall(rng)
-    tdt = "float16" if args.tables == "f16" else None
+    tdt = "float16" if tables == "f16" else None
     Op = pylops_amd.Kirchhoff(dtype="float32", tabledtype=tdt, **kw)
     plan = Op.plan
     L = _lib.load()
-    import ctypes
     pp = ctypes.byref(plan)
     st = _lib.stream_ptr(dev)
     mt = torch.as_tensor(np.asarray(refl, dtype=np.float32).ravel(), device=dev)
@@ -316,65 +340,160 @@ def main():
         if ev is not None:
             ev[3].record()
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    t_spread = np.mean([e[0].elapsed_time(e[1]) for e in evs]) * 1e-3
-    t_conv = np.mean([e[1].elapsed_time(e[2]) for e in evs]) * 1e-3
-    t_gather = np.mean([e[2].elapsed_time(e[3]) for e in evs]) * 1e-3
-    ms_per_step = wall / args.steps * 1e3
+    t_spread = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) * 1e-3
+    t_conv = float(np.mean([e[1].elapsed_time(e[2]) for e in evs])) * 1e-3
+    t_gather = float(np.mean([e[2].elapsed_time(e[3]) for e in evs])) * 1e-3
     trip = float(Op.ni) * Op.ns * Op.nr
-    value = 2.0 * trip / (wall / args.steps) / 1e9
+    res = {"tables": tables, "ms_per_step": round(wall / steps * 1e3, 4),
+           "gsamples_s": round(2.0 * trip / (wall / steps) / 1e9, 3),
+           "data_gsamples_s": round(2.0 * nd / (wall / steps) / 1e9, 4),
+           "kernel_ms": {"spread": round(t_spread * 1e3, 4),
+                         "conv_fwd+adj": round(t_conv * 1e3, 4),
+                         "gather": round(t_gather * 1e3, 4)},
+           "launch_shapes": {"rblock": int(plan.rblock), "sorder": int(plan.sorder),
+                             "sside": int(plan.sside), "gsplit": int(plan.gsplit),
+                             "gtiles": int(plan.gtiles), "tile": list(Op.tile)},
+           "compulsory_bytes": int(compulsory_bytes(Op))}
+    return res, Op, dobs
 
-    tb = 2 if args.tables == "f16" else 4
-    bytes_alg = algorithmic_bytes(desc, Op.ni, tb)
-    dom, t_dom = ("gather", t_gather) if t_gather >= t_spread else ("spread", t_spread)
-    achieved = bytes_alg / t_dom / 1e9
+
+def roofline(res, desc, ni, traffic_key):
+    """Roofline of the dominant kernel of a step: achieved = compulsory
+    bytes (every table element, base and weight once, image and traces
+    once) over the kernel's mean HIP-event time, against the 8 TB/s HBM
+    peak; the separable-stream model (receiver table once per source) and
+    the rocprofv3 counter traffic beside it."""
+    km = res["kernel_ms"]
+    dom = "gather" if km["gather"] >= km["spread"] else "spread"
+    t = km[dom] * 1e-3
+    comp = res["compulsory_bytes"]
+    sep = algorithmic_bytes(desc, ni, 2 if res["tables"] == "f16" else 4)
+    traffic, src = measured_traffic(traffic_key, dom, res["tables"])
+    out = {"bound": "hbm", "kernel": dom, "achieved": round(comp / t / 1e9, 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(comp / t / 1e9 / HBM_PEAK_GBS, 4),
+           "traffic": traffic, "model": "compulsory", "model_bytes": int(comp),
+           "separable_stream_bytes": int(sep),
+           "separable_stream_frac": round(sep / t / 1e9 / HBM_PEAK_GBS, 4)}
+    if traffic is not None:
+        out["traffic_frac"] = round(traffic / t / 1e9 / HBM_PEAK_GBS, 4)
+        out["traffic_source"] = src
+    return out
+
+
+def operator_line(dev, args, name, cpu=True, precision=True, solve=True):
+    """The bench line of one operator workload: float32 tables (the
+    headline value) and float16 offset tables beside them, solver times,
+    errors against the float64 operator and the CPU baseline."""
+    import torch
+    kw, refl, desc = build_workload(name, dev)
+    r32, Op, dobs = time_operator(kw, refl, "f32", dev, args.steps, args.warmup)
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": r32["gsamples_s"],
         "unit": "Gsamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": r32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {**desc, "tables": args.tables, "step": "forward of K^T d + adjoint of d",
-                   "kernel_ms": {"spread": round(t_spread * 1e3, 4),
-                                 "conv_fwd+adj": round(t_conv * 1e3, 4),
-                                 "gather": round(t_gather * 1e3, 4)}},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (measured_traffic(args.workload, dom)
-                                 if args.tables == "f32" else None),
-                     "model_bytes": int(bytes_alg)},
+        "config": {**desc, "tables": "f32",
+                   "step": "forward (spread + wavelet) of K^T d + adjoint (correlation + "
+                           "gather) of d",
+                   "kernel_ms": r32["kernel_ms"], "launch_shapes": r32["launch_shapes"],
+                   "data_gsamples_s": r32["data_gsamples_s"]},
+        "roofline": roofline(r32, desc, Op.ni, name),
     }
-    if not args.no_solve:
-        out["solve"] = time_solvers(Op, dobs)
-    if not args.no_solve:
-        out["precision"] = precision_report(dev, "2d", niter=10)
-    if args.extra:
-        out["extra"] = precision_report(dev, "3d", niter=20)
-    if not args.no_batch:
-        out["batch"] = time_batch(dev, sequential=not args.no_solve)[0]
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(kw, desc, refl, dobs.cpu().numpy())
+    niter = 20 if name == "3d" else (30 if name == "2d" else 10)
+    if solve and not args.no_solve:
+        out["solve"] = {"f32": time_solvers(Op, dobs, niter=niter,
+                                            which=("cgls",) if name == "3d" else ("lsqr", "cgls"))}
+    cpu_line = None
+    if cpu and not args.no_cpu:
+        cpu_line = cpu_baseline(kw, desc, refl, dobs.cpu().numpy())
+    del Op, dobs
+    torch.cuda.empty_cache()
+    if name != "2dsmall":
+        r16, Op16, dobs16 = time_operator(kw, refl, "f16", dev, args.steps, args.warmup)
+        out["config"]["fp16_tables"] = {
+            **r16, "speedup_vs_fp32": round(r32["ms_per_step"] / r16["ms_per_step"], 3),
+            "spread_speedup": round(r32["kernel_ms"]["spread"] / r16["kernel_ms"]["spread"], 3),
+            "gather_speedup": round(r32["kernel_ms"]["gather"] / r16["kernel_ms"]["gather"], 3),
+            "roofline": roofline(r16, desc, Op16.ni, name)}
+        if solve and not args.no_solve:
+            out["solve"]["f16"] = time_solvers(
+                Op16, dobs16, niter=niter, which=("cgls",) if name == "3d" else ("lsqr", "cgls"))
+        del Op16, dobs16
+        torch.cuda.empty_cache()
+    if precision and not args.no_solve and name != "2dsmall":
+        out["precision"] = precision_report(dev, name, niter=niter)
+    if cpu_line is not None:
+        out["cpu_baseline"] = cpu_line
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="3d", choices=["3d", "2d", "2dsmall", "batch", "autograd"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the solver timings and the precision reports (both run solvers)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads' lines (2D dynamic, 64-problem batch) that "
+                         "the 3d line carries under 'extra'")
+    ap.add_argument("--adam-steps", type=int, default=50,
+                    help="Adam steps of the autograd workload")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 4 GiB device copy first (reference kernel for "
+                         "FETCH_SIZE / WRITE_SIZE counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+
+    import torch
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    if args.calib_copy:
+        src = torch.ones(1024 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.workload == "batch":
+        print(json.dumps(batch_line(dev, args)), flush=True)
+        return
+    if args.workload == "autograd":
+        print(json.dumps(autograd_line(dev, args, nsteps=args.adam_steps)), flush=True)
+        return
+    out = operator_line(dev, args, args.workload)
+    if args.workload == "3d" and not args.no_extra:
+        out["extra"] = {"2d": operator_line(dev, args, "2d", cpu=False)}
+        if not args.no_solve:
+            out["extra"]["batch"] = time_batch(dev)[0]
     print(json.dumps(out), flush=True)
 
 
-def time_solvers(Op, dobs, niter=30, runs=3):
-    """Wall time of a CGLS and an LSQR solve of `niter` iterations on the
+def time_solvers(Op, dobs, niter=30, runs=3, which=("cgls", "lsqr")):
+    """Wall time of CGLS and / or LSQR solves of `niter` iterations on the
     observed data, inputs resident on the device, iterations replayed from
     a HIP graph (tol = atol = btol = 0, so every iteration runs): the median
     of `runs` solves, each run listed (the first timed CGLS solve of a
@@ -385,6 +504,8 @@ def time_solvers(Op, dobs, niter=30, runs=3):
     res = {"niter": niter}
     for name, fn, kw in (("cgls", cgls, dict(tol=0.0)),
                          ("lsqr", lsqr, dict(atol=0.0, btol=0.0, calc_var=False))):
+        if name not in which:
+            continue
         fn(Op, dobs, niter=10, graph=True, **kw)  # past the replay loop's first flag check
         torch.cuda.synchronize()
         times = []
@@ -583,8 +704,8 @@ def batch_line(dev, args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(res["workload"], dom),
-                     "model_bytes": res["model_bytes"]},
+                     "traffic": measured_traffic(res["workload"], dom, "f16")[0],
+                     "model": "separable-stream", "model_bytes": res["model_bytes"]},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_batch(probs)

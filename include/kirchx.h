@@ -42,17 +42,19 @@
 extern "C" {
 #endif
 
-#define KX_ABI_VERSION 2
+#define KX_ABI_VERSION 3
 
 /* Bytes of the per-plan device scratch `aux` (forward accumulation scale). */
 #define KX_AUX_BYTES 4096
 
 /* lane order of the forward (kx_plan.sorder): at step r of a tile the lane
- * of point p treats receiver r (natural) or (p + r) mod rblock (rotated) of
- * its workgroup's block */
+ * of point p treats receiver r (natural), (p + r) mod rblock (rotated) or
+ * (p / 16 + r) mod rblock (grouped) of its workgroup's block; every order
+ * gives bitwise the same traces */
 #define KX_ORDER_AUTO 0
 #define KX_ORDER_NATURAL 1
 #define KX_ORDER_ROTATED 2
+#define KX_ORDER_GROUPED 3
 
 /* what a forward workgroup blocks (kx_plan.sside): one source and a block of
  * rblock receivers, or one receiver and a block of rblock sources (float32
@@ -114,6 +116,10 @@ typedef struct kx_plan {
     int32_t gsplit; /* waves per tile in the adjoint, 1, 2 or 4 (each sums a
                        share of the sources), 0 = automatic */
     int32_t sside;  /* forward workgroup blocks KX_SIDE_*, 0 = automatic */
+    int32_t gtiles; /* neighbouring image tiles per wave in the adjoint (1, 2
+                       or 4; they share the staged trace windows),
+                       0 = automatic */
+    int32_t pad0;
 } kx_plan;
 
 int kx_version(void);
@@ -365,8 +371,10 @@ int kx_cgls_scalar_batch(double *sc, int32_t nseg, double *cost, double *cost1, 
  * (unit slowness gives the distance, skfmm.distance).  Fast sweeping with
  * Godunov updates: first order until no value changes, then, for order 2
  * (scikit-fmm's default), second-order one-sided differences until no value
- * changes; each phase runs at most max_iter rounds of all sweep directions,
- * and *iters (host) receives the rounds run.  2D: ndims = 2 and ny is
+ * changes; each phase runs at most max_iter rounds of all sweep directions.
+ * iters (host, two int32): iters[0] receives the rounds run, iters[1] 1 when
+ * every phase ended with no value changing and 0 when a phase ran out of
+ * rounds (the tables are then upper bounds of the converged times).  2D: ndims = 2 and ny is
  * ignored (iy = 0).  src (device) is copied back and every cell checked
  * against the grid before any launch (KX_ESHAPE).  flag: one device int32
  * of scratch.  Synchronises the stream (not capturable). */

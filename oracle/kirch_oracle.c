@@ -328,3 +328,15 @@ int or_num_threads(void)
     return 1;
 #endif
 }
+
+/* Threads of the following parallel regions (the timed CPU baseline runs
+ * at all host cores and at one thread). */
+void or_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    extern void omp_set_num_threads(int);
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -79,6 +79,8 @@ def _declare(L):
     L.or_conv_same_f64.restype = None
     L.or_num_threads.argtypes = []
     L.or_num_threads.restype = _i
+    L.or_set_num_threads.argtypes = [_i]
+    L.or_set_num_threads.restype = None
 
 
 def _ptr(a):

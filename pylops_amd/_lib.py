@@ -12,10 +12,10 @@ import torch  # noqa: F401  (must precede the HIP library)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libkirchx.so")
 
-KX_ABI_VERSION = 2
+KX_ABI_VERSION = 3
 KX_AUX_BYTES = 4096
 KX_F32, KX_F64, KX_F16 = 0, 1, 2
-KX_ORDER_AUTO, KX_ORDER_NATURAL, KX_ORDER_ROTATED = 0, 1, 2
+KX_ORDER_AUTO, KX_ORDER_NATURAL, KX_ORDER_ROTATED, KX_ORDER_GROUPED = 0, 1, 2, 3
 KX_SIDE_AUTO, KX_SIDE_RECEIVERS, KX_SIDE_SOURCES = 0, 1, 2
 KX_EINVAL, KX_ESHAPE, KX_ETYPE, KX_EABI, KX_ELDS = -1, -2, -3, -4, -5
 
@@ -127,6 +127,8 @@ class KxPlan(ctypes.Structure):
         ("aux", ctypes.c_void_p),
         ("gsplit", ctypes.c_int32),
         ("sside", ctypes.c_int32),
+        ("gtiles", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
     ]
 
 
@@ -213,7 +215,7 @@ def _declare(L):
         "kx_seg_dot": ([PS, I32, P, P, P, I32, I32, I32, P, P], ctypes.c_int),
         "kx_cgls_scalar_batch": ([P, I32, P, P, I32, I32, P], ctypes.c_int),
         "kx_eikonal": ([I32, I32, I32, I32, D, D, D, P, I32, P, P, I32, I32, P,
-                        ctypes.POINTER(I32), P], ctypes.c_int),
+                        ctypes.POINTER(I32), P], ctypes.c_int),  # iters: int32[2]
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

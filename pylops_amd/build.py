@@ -1,14 +1,19 @@
 """Build libkirchx.so (the HIP library behind pylops_amd) for gfx950.
 
-    python -m pylops_amd.build
+    python -m pylops_amd.build [extra hipcc flags]
 
-Compiles pylops_amd/csrc/*.hip with hipcc into pylops_amd/libkirchx.so.
+Compiles each pylops_amd/csrc/*.hip translation unit with hipcc into an
+object file (in parallel; every unit holds its own device code, so no
+relocatable device code is needed) and links them into
+pylops_amd/libkirchx.so.
 """
+import concurrent.futures as cf
 import glob
 import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -19,7 +24,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # packed fp32 arithmetic is switched off for every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
@@ -38,15 +42,30 @@ def sources():
     return sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
 
 
-def build(verbose=True, extra=()):
-    tmp = LIB + ".tmp"
+def _compile(src, obj, extra, verbose):
     cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-I", os.path.join(ROOT, "include"),
-           *sources(), "-o", tmp]
+           "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    return obj
+
+
+def build(verbose=True, extra=(), out=LIB, jobs=None):
+    """Compile every source and link ``out``; returns its path."""
+    srcs = sources()
+    jobs = jobs or min(len(srcs), os.cpu_count() or 1, 8)
+    with tempfile.TemporaryDirectory(prefix="kx_build_") as tmpd:
+        objs = [os.path.join(tmpd, os.path.basename(s) + ".o") for s in srcs]
+        with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+            list(ex.map(lambda so: _compile(so[0], so[1], extra, verbose), zip(srcs, objs)))
+        tmp = out + ".tmp"
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

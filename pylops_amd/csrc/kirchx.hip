@@ -86,14 +86,14 @@ extern "C" const char *kx_last_error(void) { return g_err; }
 // ---------------------------------------------------------------------------
 namespace {
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB, bool XS>
+template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int RB, bool XS>
 __global__ __launch_bounds__(kSpreadBlock) void spread_kernel(KArgs a, const TD *__restrict__ m,
                                                               TD *__restrict__ y,
                                                               const SpreadScale *__restrict__ sc)
 {
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const SpreadScale scale = *sc;
-    spread_item<TD, TT, DYN, NEG, ROT, RB, XS>(a, m, y, scale, L % a.ns, (L / a.ns) * RB);
+    spread_item<TD, TT, DYN, NEG, ORD, RB, XS>(a, m, y, scale, L % a.ns, (L / a.ns) * RB);
 }
 
 // K models of a float32 operator; workgroup L (after the XCD renumbering) =
@@ -116,14 +116,14 @@ __global__ __launch_bounds__(kSpreadBlock) void spread_models_kernel(
 }
 
 // K models (K > 1: model k's traces at d + k*dstride, its image at
-// m + k*mstride)
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int K>
+// m + k*mstride); groups of NP tiles per wave
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP, int K>
 __global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD *__restrict__ d,
                                                               TD *__restrict__ m,
                                                               int64_t dstride, int64_t mstride)
 {
-    gather_tiles<TD, TT, DYN, NEG, JW, P, K>(a, d, m, xcd_remap(blockIdx.x, gridDim.x), dstride,
-                                             mstride);
+    gather_tiles<TD, TT, DYN, NEG, JW, P, NP, K>(a, d, m, xcd_remap(blockIdx.x, gridDim.x),
+                                                 dstride, mstride);
 }
 
 template <typename TD>
@@ -304,34 +304,35 @@ constexpr int kMaxWavesPerCu = 32;
 
 size_t spread_lds_bytes(int rb, int nt) { return (size_t)rb * nt * 8; }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT, int RB>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD, int RB>
 int launch_spread_rb(const KArgs &a, const void *m, void *y, const SpreadScale *sc,
                      hipStream_t st)
 {
     const size_t lds = spread_lds_bytes(RB, a.nt);
     const int nrb = (a.nr + RB - 1) / RB;
     const int nblk = a.ns * nrb;
-    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, ROT, RB, XS>), dim3(nblk),
+    allow_lds(reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, RB, XS>), lds);
+    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, ORD, RB, XS>), dim3(nblk),
                        dim3(kSpreadBlock), lds, st, a, reinterpret_cast<const TD *>(m),
                        reinterpret_cast<TD *>(y), sc);
     KX_LAUNCHED("spread");
     return KX_OK;
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD>
 const void *spread_fn(int rb)
 {
     switch (rb) {
     case 16:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 16, XS>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 16, XS>);
     case 8:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 8, XS>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 8, XS>);
     case 4:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 4, XS>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 4, XS>);
     case 2:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ROT, 2, XS>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 2, XS>);
     default:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, false, 1, XS>);
+        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, kOrdNatural, 1, XS>);
     }
 }
 
@@ -340,6 +341,7 @@ const void *spread_fn(int rb)
 int spread_per_cu(const void *fn, int rb, int nt)
 {
     const size_t lds = spread_lds_bytes(rb, nt);
+    allow_lds(fn, lds);
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kSpreadBlock, lds) != hipSuccess ||
         n <= 0) {
@@ -354,11 +356,11 @@ int spread_per_cu(const void *fn, int rb, int nt)
 // traces resident per CU, weighted by how far the resident waves (up to 24)
 // can hide the latency of the table loads; ties go to the larger block,
 // which re-reads the image and the source table less often.
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD>
 int pick_rblock(int requested, int nr, int nt, int max_rb)
 {
     auto fits = [&](int rb) {
-        return rb <= max_rb && spread_lds_bytes(rb, nt) <= (size_t)kLdsBudget;
+        return rb <= max_rb && spread_lds_bytes(rb, nt) <= lds_budget();
     };
     if (requested > 0) {
         for (int rb : kRblocks)
@@ -371,7 +373,7 @@ int pick_rblock(int requested, int nr, int nt, int max_rb)
     for (int rb : kRblocks) {
         if (!fits(rb) || (rb > 1 && rb / 2 >= nr))
             continue;
-        const int per_cu = spread_per_cu(spread_fn<TD, TT, DYN, NEG, XS, ROT>(rb), rb, nt);
+        const int per_cu = spread_per_cu(spread_fn<TD, TT, DYN, NEG, XS, ORD>(rb), rb, nt);
         const double waves = std::min(24.0, (double)per_cu * kSpreadWaves);
         const double score = waves / 24.0 * per_cu * std::min(rb, nr);
         if (score > best_score * 1.0001) {
@@ -382,22 +384,22 @@ int pick_rblock(int requested, int nr, int nt, int max_rb)
     return best;
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, bool ROT>
+template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD>
 int launch_spread_o(KArgs a, int requested, int max_rb, const void *m, void *y,
                     const SpreadScale *sc, hipStream_t st)
 {
-    a.rblock = pick_rblock<TD, TT, DYN, NEG, XS, ROT>(requested, a.nr, a.nt, max_rb);
+    a.rblock = pick_rblock<TD, TT, DYN, NEG, XS, ORD>(requested, a.nr, a.nt, max_rb);
     switch (a.rblock) {
     case 16:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 16>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 16>(a, m, y, sc, st);
     case 8:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 8>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 8>(a, m, y, sc, st);
     case 4:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 4>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 4>(a, m, y, sc, st);
     case 2:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ROT, 2>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 2>(a, m, y, sc, st);
     default:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, false, 1>(a, m, y, sc, st);
+        return launch_spread_rb<TD, TT, DYN, NEG, XS, kOrdNatural, 1>(a, m, y, sc, st);
     }
 }
 
@@ -405,16 +407,24 @@ template <typename TD, typename TT, bool DYN, bool NEG, bool XS>
 int launch_spread_t(const KArgs &a, int order, int requested, int max_rb, const void *m, void *y,
                     const SpreadScale *sc, hipStream_t st)
 {
-    return order == KX_ORDER_ROTATED
-               ? launch_spread_o<TD, TT, DYN, NEG, XS, true>(a, requested, max_rb, m, y, sc, st)
-               : launch_spread_o<TD, TT, DYN, NEG, XS, false>(a, requested, max_rb, m, y, sc, st);
+    switch (order) {
+    case KX_ORDER_ROTATED:
+        return launch_spread_o<TD, TT, DYN, NEG, XS, kOrdRotated>(a, requested, max_rb, m, y, sc,
+                                                                  st);
+    case KX_ORDER_GROUPED:
+        return launch_spread_o<TD, TT, DYN, NEG, XS, kOrdGrouped>(a, requested, max_rb, m, y, sc,
+                                                                  st);
+    default:
+        return launch_spread_o<TD, TT, DYN, NEG, XS, kOrdNatural>(a, requested, max_rb, m, y, sc,
+                                                                  st);
+    }
 }
 
 
 template <typename TD, typename TT>
 int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream_t st)
 {
-    if (spread_lds_bytes(1, p->nt) > (size_t)kLdsBudget)
+    if (spread_lds_bytes(1, p->nt) > lds_budget())
         return kx_fail(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
     if (sizeof(TD) == 4 && !p->aux)
         return kx_fail(KX_EINVAL, "spread: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
@@ -472,68 +482,116 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
                                                                       max_rb, m, y, sc, st);
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int P, int K>
-int launch_gather_p(const KArgs &a, const void *d, int64_t dstride, void *m, int64_t mstride,
+// sources per wave and pass (measured): every pass re-reads the receiver
+// table -- three tables when dynamic -- and more sources hold more
+// registers, lowering occupancy
+template <bool DYN, int NP, int K> struct GatherJW {
+    static constexpr int v = (DYN && NP == 1 && K == 1) ? 8 : 4;
+};
+
+template <typename TD, typename TT, bool DYN, bool NEG, int P, int NP, int K>
+int launch_gather_p(KArgs a, const void *d, int64_t dstride, void *m, int64_t mstride,
                     hipStream_t st)
 {
-    // sources per wave and pass (measured): every pass re-reads the receiver
-    // table -- three tables when dynamic -- and more sources hold more
-    // registers, lowering occupancy
-    constexpr int JW = DYN ? 8 : 4;
-    constexpr int NT = kGatherWaves / P;
+    constexpr int JW = GatherJW<DYN, NP, K>::v;
+    constexpr int NG = kGatherWaves / P;
     if ((int64_t)JW * P * a.nr * a.nt >= ((int64_t)1 << 31))
         return kx_fail(KX_ESHAPE, "gather: %d sources x nr x nt exceeds 2^31 samples", JW * P);
-    const int nblk = (a.ntiles + NT - 1) / NT;
-    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P, K>), dim3(nblk),
+    set_tile_groups(a, NP);
+    const int nblk = (a.ngroups + NG - 1) / NG;
+    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P, NP, K>), dim3(nblk),
                        dim3(kGatherBlock), 0, st, a, reinterpret_cast<const TD *>(d),
                        reinterpret_cast<TD *>(m), dstride, mstride);
     KX_LAUNCHED("gather");
     return KX_OK;
 }
 
-// Waves per tile: the fewest that give every CU kGatherFill waves.
+// Waves per tile group: the fewest that give every CU kGatherFill waves.
 constexpr int kGatherFill = 24;
 
-int gather_split(const KArgs &a, int gsplit)
+int device_cus()
+{
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    return cus;
+}
+
+// Tiles per wave (NP): requested (rounded down to 1, 2 or 4), or the most
+// whose groups still give every CU kGatherFill waves with at most
+// kGatherWaves waves per group.
+int gather_tiles_per_wave(const KArgs &a, int gtiles)
+{
+    if (gtiles > 0)
+        return gtiles >= 4 ? 4 : (gtiles >= 2 ? 2 : 1);
+    const int64_t want = (int64_t)kGatherFill * device_cus();
+    for (int np : {4, 2}) {
+        KArgs b = a;
+        set_tile_groups(b, np);
+        if ((int64_t)b.ngroups * kGatherWaves >= want)
+            return np;
+    }
+    return 1;
+}
+
+int gather_split(const KArgs &a, int np, int gsplit)
 {
     int P = 1;
     if (gsplit > 0) {
         while (P * 2 <= std::min(gsplit, kGatherWaves))
             P *= 2;
     } else {
-        int cus = 256, dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            cus = 256;
-        while (P < kGatherWaves && (int64_t)a.ntiles * P < (int64_t)kGatherFill * cus)
+        KArgs b = a;
+        set_tile_groups(b, np);
+        const int64_t want = (int64_t)kGatherFill * device_cus();
+        while (P < kGatherWaves && (int64_t)b.ngroups * P < want)
             P *= 2;
     }
     return P;
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int K = 1>
-int launch_gather_t(const KArgs &a, int gsplit, const void *d, void *m, hipStream_t st,
-                    int64_t dstride = 0, int64_t mstride = 0)
+template <typename TD, typename TT, bool DYN, bool NEG, int NP, int K>
+int launch_gather_np(const KArgs &a, int P, const void *d, void *m, hipStream_t st,
+                     int64_t dstride, int64_t mstride)
 {
-    switch (gather_split(a, gsplit)) {
+    switch (P) {
     case 1:
-        return launch_gather_p<TD, TT, DYN, NEG, 1, K>(a, d, dstride, m, mstride, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 1, NP, K>(a, d, dstride, m, mstride, st);
     case 2:
-        return launch_gather_p<TD, TT, DYN, NEG, 2, K>(a, d, dstride, m, mstride, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 2, NP, K>(a, d, dstride, m, mstride, st);
     default:
-        return launch_gather_p<TD, TT, DYN, NEG, 4, K>(a, d, dstride, m, mstride, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 4, NP, K>(a, d, dstride, m, mstride, st);
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, int K = 1>
+int launch_gather_t(const KArgs &a, int gsplit, int gtiles, const void *d, void *m,
+                    hipStream_t st, int64_t dstride = 0, int64_t mstride = 0)
+{
+    const int np = gather_tiles_per_wave(a, gtiles);
+    const int P = gather_split(a, np, gsplit);
+    switch (np) {
+    case 4:
+        return launch_gather_np<TD, TT, DYN, NEG, 4, K>(a, P, d, m, st, dstride, mstride);
+    case 2:
+        return launch_gather_np<TD, TT, DYN, NEG, 2, K>(a, P, d, m, st, dstride, mstride);
+    default:
+        return launch_gather_np<TD, TT, DYN, NEG, 1, K>(a, P, d, m, st, dstride, mstride);
     }
 }
 
 template <typename TD, typename TT>
 int launch_gather_d(const kx_plan *p, const KArgs &a, const void *d, void *m, hipStream_t st)
 {
-    const int g = p->gsplit;
+    const int g = p->gsplit, n = p->gtiles;
     if (p->dynamic)
-        return p->neg_trav ? launch_gather_t<TD, TT, true, true>(a, g, d, m, st)
-                           : launch_gather_t<TD, TT, true, false>(a, g, d, m, st);
-    return p->neg_trav ? launch_gather_t<TD, TT, false, true>(a, g, d, m, st)
-                       : launch_gather_t<TD, TT, false, false>(a, g, d, m, st);
+        return p->neg_trav ? launch_gather_t<TD, TT, true, true>(a, g, n, d, m, st)
+                           : launch_gather_t<TD, TT, true, false>(a, g, n, d, m, st);
+    return p->neg_trav ? launch_gather_t<TD, TT, false, true>(a, g, n, d, m, st)
+                       : launch_gather_t<TD, TT, false, false>(a, g, n, d, m, st);
 }
 
 // K float32 models per launch, K in {4, 2}, then one by one.
@@ -543,20 +601,20 @@ int launch_gather_models(const kx_plan *p, const KArgs &a, const float *d, float
 {
     const int64_t ni = (int64_t)p->nx * p->ny * p->nz;
     const int64_t dstride = (int64_t)p->ns * p->nr * p->nt;
-    const int g = p->gsplit;
+    const int g = p->gsplit, n = p->gtiles;
     int j = 0, rc = KX_OK;
     while (j < nmodel && rc == KX_OK) {
         const int left = nmodel - j;
         const float *dj = d + j * dstride;
         float *mj = m + j * ni;
         if (left >= 4) {
-            rc = launch_gather_t<float, TT, DYN, NEG, 4>(a, g, dj, mj, st, dstride, ni);
+            rc = launch_gather_t<float, TT, DYN, NEG, 4>(a, g, n, dj, mj, st, dstride, ni);
             j += 4;
         } else if (left >= 2) {
-            rc = launch_gather_t<float, TT, DYN, NEG, 2>(a, g, dj, mj, st, dstride, ni);
+            rc = launch_gather_t<float, TT, DYN, NEG, 2>(a, g, n, dj, mj, st, dstride, ni);
             j += 2;
         } else {
-            rc = launch_gather_t<float, TT, DYN, NEG, 1>(a, g, dj, mj, st);
+            rc = launch_gather_t<float, TT, DYN, NEG, 1>(a, g, n, dj, mj, st);
             j += 1;
         }
     }

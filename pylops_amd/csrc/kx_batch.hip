@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kSpreadBlock) void batch_spread_kernel(
         const SpreadItem w = items[it];
         const KArgs a = probs[w.p].a;
         const SpreadScale sc = scales[w.p];
-        spread_item<TD, TT, DYN, NEG, ROT, RB>(a, m + probs[w.p].moff, y + probs[w.p].doff, sc,
+        spread_item<TD, TT, DYN, NEG, ROT ? kOrdRotated : kOrdNatural, RB>(a, m + probs[w.p].moff, y + probs[w.p].doff, sc,
                                                w.s, w.r0);
     }
     launch_done(ctr);
@@ -177,7 +177,8 @@ __global__ __launch_bounds__(kGatherBlock) void batch_gather_kernel(
             break;
         const GatherItem w = items[it];
         const KArgs a = probs[w.p].a;
-        gather_tiles<TD, TT, DYN, NEG, JW, P>(a, d + probs[w.p].doff, m + probs[w.p].moff, w.L);
+        gather_tiles<TD, TT, DYN, NEG, JW, P, 1>(a, d + probs[w.p].doff, m + probs[w.p].moff,
+                                                 w.L);
     }
     launch_done(ctr);
 }

@@ -17,7 +17,40 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr int kLdsBudget = 64 * 1024;
+constexpr int kLdsBudget = 64 * 1024;  // LDS of one workgroup on every CDNA device
+
+// LDS one workgroup of the current device may use: the device's limit
+// (160 KB on gfx950), queried once per device, or kLdsBudget without one.
+inline size_t lds_budget()
+{
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return kLdsBudget;
+    }
+    if (cached[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess ||
+            v < kLdsBudget) {
+            (void)hipGetLastError();
+            v = kLdsBudget;
+        }
+        cached[dev] = v;
+    }
+    return (size_t)cached[dev];
+}
+
+// Lets kernel fn take `lds` bytes of dynamic LDS (beyond 64 KB the
+// attribute must be raised first).
+inline void allow_lds(const void *fn, size_t lds)
+{
+    if (lds > (size_t)kLdsBudget &&
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+        (void)hipGetLastError();
+}
 constexpr int kReduceBlocks = 1024;
 
 template <typename TD> struct TimeType { using type = float; };
@@ -49,6 +82,9 @@ struct KArgs {
     int ntz, ntx, nty;
     int ntiles;
     int rblock;
+    // adjoint tile groups: gx x gy tiles, ngx x ngy groups per z row,
+    // ngroups in all (set_tile_groups; 1 x 1 by default)
+    int gx, gy, ngx, ngy, ngroups;
     const void *off_s;
     const int *base_s;
     const void *off_r;
@@ -78,6 +114,10 @@ inline KArgs make_args(const kx_plan *p)
     a.nty = p->nty;
     a.ntiles = (int)p->ntiles;
     a.rblock = p->rblock;
+    a.gx = a.gy = 1;
+    a.ngx = p->ntx;
+    a.ngy = p->nty;
+    a.ngroups = (int)p->ntiles;
     a.off_s = p->off_s;
     a.base_s = p->base_s;
     a.off_r = p->off_r;
@@ -206,44 +246,60 @@ __device__ __forceinline__ int vgpr_zero()
 
 // Lane roles.  Lane p holds point p of the tile and reads the source side
 // of its point once per tile.  At step r (r < RB) it treats receiver j of
-// the workgroup's block: j = r in natural order, so that the table loads of
-// a step are coalesced rows, or j = (p + r) mod RB in rotated order, so that
-// one atomic instruction covers all RB trace rows at 64 / RB points RB apart
-// each.  Samples of neighbouring points of one trace often coincide, and the
-// lanes of an LDS atomic that hit one address are serialised: the rotated
-// order trades scattered table loads for fewer such collisions.
-template <int RB, bool ROT> __device__ __forceinline__ int step_receiver(int lane, int r)
+// the workgroup's block:
+//   natural  j = r: the table loads of a step are coalesced rows;
+//   rotated  j = (p + r) mod RB: one atomic instruction covers all RB trace
+//            rows at 64 / RB points RB apart each;
+//   grouped  j = (p / 16 + r) mod RB: each group of 16 lanes (a run of 16
+//            points along z in the default tiles) treats one receiver, so a
+//            load reads four 64-byte row segments, as many as a coalesced
+//            row, and one atomic instruction covers four trace rows.
+// Samples of neighbouring points of one trace often coincide, and the lanes
+// of an LDS atomic that hit one address are serialised: the rotated order
+// trades scattered table loads for fewer such collisions, the grouped order
+// keeps the loads of the natural one and splits the collisions four ways.
+constexpr int kOrdNatural = 0;
+constexpr int kOrdRotated = 1;
+constexpr int kOrdGrouped = 2;
+template <int RB, int ORD> __device__ __forceinline__ int step_receiver(int lane, int r)
 {
-    return ROT ? (lane + r) & (RB - 1) : r;
+    if constexpr (ORD == kOrdRotated)
+        return (lane + r) & (RB - 1);
+    else if constexpr (ORD == kOrdGrouped)
+        return ((lane >> 4) + r) & (RB - 1);
+    else
+        return r;
 }
 
 // Receiver bases held per lane: the base of receiver (lane mod RB) of the
 // block, one load for all RB receivers, handed to the lanes that need it by
 // a lane read (natural order) or a quad permutation (rotated order, RB <= 4);
-// rotated blocks of 8 and 16 receivers load one base per step instead.
-template <bool ROT, int RB> struct BaseLoads {
-    static constexpr int n = (ROT && RB >= 8) ? RB : 1;
+// rotated blocks of 8 and 16 receivers and grouped blocks load one base per
+// step instead.
+template <int ORD, int RB> struct BaseLoads {
+    static constexpr int n =
+        ((ORD == kOrdRotated && RB >= 8) || (ORD == kOrdGrouped && RB >= 2)) ? RB : 1;
 };
 
 // One lane's share of one tile: the image value and the table entries of
 // its point, receivers in the lane's step order.
-template <typename TD, typename TT, bool ROT, int RB> struct SpreadTile {
+template <typename TD, typename TT, int ORD, int RB> struct SpreadTile {
     TD mv;
     TT os;       // raw table elements: converted at use, so that a prefetch
     TT orr[RB];  // never waits for them
     TD ps, qs;
     TD pr[RB], qr[RB];
     int bs;
-    int br[BaseLoads<ROT, RB>::n];
+    int br[BaseLoads<ORD, RB>::n];
 };
 
 // The receiver base of step r for this lane.
-template <bool ROT, int RB>
-__device__ __forceinline__ int step_base(const int (&br)[BaseLoads<ROT, RB>::n], int r)
+template <int ORD, int RB>
+__device__ __forceinline__ int step_base(const int (&br)[BaseLoads<ORD, RB>::n], int r)
 {
-    if constexpr (BaseLoads<ROT, RB>::n > 1) {
+    if constexpr (BaseLoads<ORD, RB>::n > 1) {
         return br[r];
-    } else if constexpr (!ROT || RB == 1) {
+    } else if constexpr (ORD != kOrdRotated || RB == 1) {
         // every lane treats receiver r: the base held by lane r
         return __builtin_amdgcn_readlane(br[0], r);
     } else if constexpr (RB == 2) {
@@ -265,11 +321,11 @@ __device__ __forceinline__ int step_base(const int (&br)[BaseLoads<ROT, RB>::n],
     }
 }
 
-template <typename TD, typename TT, bool DYN, bool ROT, int RB>
+template <typename TD, typename TT, bool DYN, int ORD, int RB>
 __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c, int lane, int rel,
                                             int lz, int lx, int ly, int s, int r0, int nrl,
                                             const TD *__restrict__ m, int vz,
-                                            SpreadTile<TD, TT, ROT, RB> &L)
+                                            SpreadTile<TD, TT, ORD, RB> &L)
 {
     const int t = c.t;
     const int iz = c.tzi * a.tz + lz, ix = c.txi * a.tx + lx, iy = c.tyi * a.ty + ly;
@@ -288,17 +344,17 @@ __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c,
     }
     const TT *orow = reinterpret_cast<const TT *>(a.off_r) + tr * kWave;
     const int *brow = a.base_r + tr;
-    if constexpr (BaseLoads<ROT, RB>::n == 1) {
+    if constexpr (BaseLoads<ORD, RB>::n == 1) {
         const int jl = lane & (RB - 1);
         L.br[0] = brow[(unsigned)((jl < nrl ? jl : 0) + vz)];
     }
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-        int j = step_receiver<RB, ROT>(lane, r);
+        int j = step_receiver<RB, ORD>(lane, r);
         j = j < nrl ? j : 0;  // rows past the last receiver read row 0
         const unsigned e = (unsigned)(j * kWave + lane);
         L.orr[r] = orow[e];
-        if constexpr (BaseLoads<ROT, RB>::n > 1)
+        if constexpr (BaseLoads<ORD, RB>::n > 1)
             L.br[r] = brow[(unsigned)(j + vz)];
         if constexpr (DYN) {
             L.pr[r] = (reinterpret_cast<const TD *>(a.wp_r) + tr * kWave)[e];
@@ -377,9 +433,9 @@ __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int
 // products formed once per tile; with the sides exchanged (XS: the block
 // members are sources) the same products are formed per step, so that the
 // weights round as they do with blocks of receivers.
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB,
+template <typename TD, typename TT, bool DYN, bool NEG, int ORD, bool FIX, int RB,
           bool XS = false>
-__device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, RB> &cur, int iz,
+__device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, RB> &cur, int iz,
                                                   int e,
                                                   const int (&lrow)[RB], const int (&izm)[RB],
                                                   unsigned ntm1, double *accd,
@@ -410,7 +466,7 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ROT, 
                 w = fma(mqs, cur.qr[r], mps * cur.pr[r]);
             on = iz >= izm[r];
         }
-        spread_sample<TD, TC, NEG, FIX>(T, cur.bs + step_base<ROT, RB>(cur.br, r), w, on,
+        spread_sample<TD, TC, NEG, FIX>(T, cur.bs + step_base<ORD, RB>(cur.br, r), w, on,
                                         lrow[r], ntm1, accd, acci);
     }
 }
@@ -450,7 +506,7 @@ __device__ __forceinline__ void traces_store(TD *__restrict__ y, size_t first, i
     __syncthreads();  // the traces are read out before a next item clears them
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, bool FIX, int RB,
+template <typename TD, typename TT, bool DYN, bool NEG, int ORD, bool FIX, int RB,
           bool XS = false>
 __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, int e, int s, int r0)
@@ -475,7 +531,7 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     int izm[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-        const int j = step_receiver<RB, ROT>(lane, r);
+        const int j = step_receiver<RB, ORD>(lane, r);
         lrow[r] = j < nrl ? j * a.nt : -1;
         if constexpr (DYN)  // the gate of a pair is indexed like its trace
             izm[r] = a.izmin[(size_t)s * a.trs + (size_t)(r0 + (j < nrl ? j : 0)) * a.trb];
@@ -485,17 +541,17 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     // the next tile is loaded while this one is accumulated (a twice
     // unrolled loop with two buffers and no copies measured slower: the
     // second buffer costs registers and occupancy)
-    SpreadTile<TD, TT, ROT, RB> cur, nxt;
+    SpreadTile<TD, TT, ORD, RB> cur, nxt;
     TileCursor c;
     c.start(a, wave);
     if (c.t < a.ntiles)
-        spread_load<TD, TT, DYN, ROT, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, cur);
+        spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, cur);
     while (c.t < a.ntiles) {
         TileCursor cn = c;
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
-            spread_load<TD, TT, DYN, ROT, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
-        spread_accumulate<TD, TT, DYN, NEG, ROT, FIX, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
+            spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
+        spread_accumulate<TD, TT, DYN, NEG, ORD, FIX, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
                                                               izm, ntm1, accd, acci);
         cur = nxt;
         c = cn;
@@ -505,18 +561,18 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
 }
 
 // One (source, receiver block) item with the launch's accumulation scale.
-template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB, bool XS = false>
+template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int RB, bool XS = false>
 __device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, const SpreadScale &sc, int s,
                                             int r0)
 {
     if constexpr (sizeof(TD) == 4) {
         if (sc.fixed)
-            spread_body<TD, TT, DYN, NEG, ROT, true, RB, XS>(a, m, y, sc.e, s, r0);
+            spread_body<TD, TT, DYN, NEG, ORD, true, RB, XS>(a, m, y, sc.e, s, r0);
         else
-            spread_body<TD, TT, DYN, NEG, ROT, false, RB, XS>(a, m, y, 0, s, r0);
+            spread_body<TD, TT, DYN, NEG, ORD, false, RB, XS>(a, m, y, 0, s, r0);
     } else {
-        spread_body<TD, TT, DYN, NEG, ROT, false, RB, XS>(a, m, y, 0, s, r0);
+        spread_body<TD, TT, DYN, NEG, ORD, false, RB, XS>(a, m, y, 0, s, r0);
     }
 }
 
@@ -755,42 +811,106 @@ __global__ __launch_bounds__(kWave) void scale_final_kernel(const float *__restr
 // ---------------------------------------------------------------------------
 // adjoint: gather per image point from wave-staged trace windows
 // ---------------------------------------------------------------------------
-// A wave owns one tile and a share of the sources (P waves of a workgroup
-// split a tile's sources; their partial sums are added in a fixed order at
-// the end).  It sweeps the receivers r; for each of its JW sources the trace
-// window of kWin samples centred on the tile's base sum (source plus
-// receiver base: the tile's own offsets span a few tens of samples) is read
-// with one coalesced load -- issued a round ahead -- into the wave's own LDS
-// region, and each lane reads its two samples there.  A 64-lane gather from
-// global memory costs the texture path ~10x the coalesced load and LDS reads
-// that replace it.  Samples outside the window are read from global memory,
-// so the window size affects speed only.  No barrier until the final sum.
+// A wave owns a group of NP neighbouring tiles (NP = 1, 2 or 4: adjacent
+// along x, or 2 x 2 in x and y; KArgs.gx x KArgs.gy tiles) and a share of
+// the sources (P waves of a workgroup split a group's sources; their partial
+// sums are added in a fixed order at the end).  It sweeps the receivers r.
+// For each of its JW sources the trace window of kWin samples starting at
+// the group's mean base sum (source plus receiver base) minus kWin / 2 is
+// staged a round ahead: lane l loads window sample l with one coalesced
+// load, and at the start of the round writes the pair (d[k], d[k+1] - d[k])
+// of window position l to the wave's own LDS region, d[k+1] coming from lane
+// l + 1 by a DPP lane shift.  Pairs outside [0, nt - 2] (the reference's
+// 0 <= it < nt - 1) and the last position (no neighbour) are zero.  A
+// point's contribution is then one 8-byte LDS read and one fma,
+// fma(f, d[k+1] - d[k], d[k]), the reference's d[k] (1 - f) + d[k+1] f.  A
+// sample outside its window reads the zero pair and is marked; the wave
+// adds the marked samples afterwards from global memory (rare: a tile's
+// offsets span a few tens of samples).  A 64-lane gather from global memory
+// costs the texture path ~10x the coalesced load and LDS read that replace
+// it, and a window serves the NP tiles of the group, so the staging is paid
+// once per NP points.  No barrier until the final sum.
 constexpr int kGatherWaves = 4;
 constexpr int kGatherBlock = kGatherWaves * kWave;
 constexpr int kWin = kWave;  // samples per staged window, one per lane
 
+// lane l + 1's value (DPP wave_shl:1; lane 63 receives 0)
+__device__ __forceinline__ float lane_next(float v)
+{
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_next(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x130, 0xf,
+                                                              0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x130,
+                                                              0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <int NP> struct Log2 {
+    static constexpr int v = NP >= 4 ? 2 : (NP >= 2 ? 1 : 0);
+};
+
+// Tile-group layout of the adjoint: groups of gx x gy tiles (gx * gy = np),
+// z fastest, then x, then y.
+inline void set_tile_groups(KArgs &a, int np)
+{
+    if (np <= 1) {
+        a.gx = a.gy = 1;
+    } else if (np == 2 || a.nty == 1) {
+        a.gx = np;
+        a.gy = 1;
+    } else {
+        a.gx = 2;
+        a.gy = np / 2;
+    }
+    a.ngx = (a.ntx + a.gx - 1) / a.gx;
+    a.ngy = (a.nty + a.gy - 1) / a.gy;
+    a.ngroups = a.ntz * a.ngx * a.ngy;
+}
+
 // K > 1: the data of K models (model k at d + k*dstride, its image at
 // m + k*mstride) share the tables, sample indices, weights and window
-// positions; each model has its own staged windows and sum, with the
+// positions; each model has its own staged windows and sums, with the
 // single-model operations in the single-model order, so each image is
 // bitwise the one K = 1 gives.
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int K = 1>
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP, int K = 1>
 __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restrict__ d,
                                              TD *__restrict__ m, int L, int64_t dstride = 0,
                                              int64_t mstride = 0)
 {
     using TC = typename TimeType<TD>::type;
-    constexpr int NT = kGatherWaves / P;  // tiles per workgroup
+    using TP = VecN<TD, 2>;               // (d[k], d[k+1] - d[k])
+    constexpr int NG = kGatherWaves / P;  // tile groups per workgroup
     constexpr int SB = JW * P;            // sources per block (all parts)
-    __shared__ TD win[kGatherWaves][K][JW][kWin];
+    constexpr int kFar = -(1 << 26);      // base of a missing source: before every trace
+    __shared__ TP win[kGatherWaves][K][JW][kWin];
+    static_assert(K * NP * kGatherBlock * sizeof(TD) <= sizeof(win), "reduction space");
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int part = wave / NT;  // this wave sums sources s0 + part + P i
-    const int tt = L * NT + wave % NT;
-    const bool active = tt < a.ntiles;
-    const int t = active ? tt : a.ntiles - 1;
-    const TilePoint p = tile_point(a, t, lane);
+    const int part = wave / NG;  // this wave sums sources s0 + part + P i
+    const int gq = L * NG + wave % NG;
+    const bool active = gq < a.ngroups;
+    const int g = active ? gq : a.ngroups - 1;
+    const int tzi = g % a.ntz;
+    const int grest = g / a.ntz;
+    const int gx0 = (grest % a.ngx) * a.gx, gy0 = (grest / a.ngx) * a.gy;
+    int t[NP];
+    bool tok[NP];
+    TilePoint pt[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const int txi = gx0 + j % a.gx, tyi = gy0 + j / a.gx;
+        tok[j] = txi < a.ntx && tyi < a.nty;
+        // a tile past the grid's edge repeats the group's first tile and
+        // stores nothing
+        t[j] = tzi + a.ntz * (tok[j] ? txi + a.ntx * tyi : gx0 + a.ntx * gy0);
+        pt[j] = tile_point(a, t[j], lane);
+    }
     const TT *offs = reinterpret_cast<const TT *>(a.off_s);
     const TT *offr = reinterpret_cast<const TT *>(a.off_r);
     const TD *wps = reinterpret_cast<const TD *>(a.wp_s);
@@ -799,173 +919,218 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
     const TD *wqr = reinterpret_cast<const TD *>(a.wq_r);
     const unsigned ntm1 = (unsigned)(a.nt - 1);
     const int vz = vgpr_zero();
-    TD(&w)[K][JW][kWin] = win[wave];
+    TP(&w)[K][JW][kWin] = win[wave];
 
-    TD acc[K];
+    TD acc[K][NP];
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        acc[k] = TD(0);
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+            acc[k][j] = TD(0);
     if (active) {
         for (int s0 = 0; s0 < a.ns; s0 += SB) {
-            // this wave's sources j = s0 + part + P i
-            TC os[JW];
-            int bs[JW];
-            TD ps[JW], qs[JW];
-            // trace (j, receiver 0) as an element offset from the block's
-            // first trace (fits 31 bits: checked at launch), held in vector
-            // registers: JW 64-bit pointers would exhaust the scalar ones
+            // this wave's sources s0 + part + P i: offsets per point, bases
+            // per tile (wave-uniform, held in scalar registers)
+            TC os[JW][NP];
+            int bs[JW][NP];
+            TD ps[JW][NP], qs[JW][NP];
+            // trace (source, receiver 0) as an element offset from the
+            // block's first trace (fits 31 bits: checked at launch), held in
+            // vector registers: JW 64-bit pointers would exhaust the scalar
+            // ones
             const TD *dblk = d + (size_t)s0 * a.nr * a.nt;
             int soff[JW], izoff[JW];
 #pragma unroll
             for (int i = 0; i < JW; ++i) {
-                const int j = s0 + part + P * i;
-                if (j < a.ns) {
-                    const size_t so = (size_t)t * a.ns + j;
-                    os[i] = (TC)ld_off(offs + so * kWave + lane);
-                    bs[i] = a.base_s[so + vz];
-                    if constexpr (DYN) {
-                        ps[i] = wps[so * kWave + lane];
-                        qs[i] = wqs[so * kWave + lane];
+                const int js = s0 + part + P * i;
+#pragma unroll
+                for (int j = 0; j < NP; ++j) {
+                    if (js < a.ns) {
+                        const size_t so = (size_t)t[j] * a.ns + js;
+                        os[i][j] = (TC)ld_off(offs + so * kWave + lane);
+                        bs[i][j] = __builtin_amdgcn_readfirstlane(a.base_s[so + vz]);
+                        if constexpr (DYN) {
+                            ps[i][j] = wps[so * kWave + lane];
+                            qs[i][j] = wqs[so * kWave + lane];
+                        }
+                    } else {  // past the last source: every sample before the trace
+                        os[i][j] = TC(0);
+                        bs[i][j] = kFar;
+                        ps[i][j] = qs[i][j] = TD(0);
                     }
-                } else {  // past the last source: every sample before the trace
-                    os[i] = TC(0);
-                    bs[i] = -(1 << 30);
-                    ps[i] = qs[i] = TD(0);
                 }
-                soff[i] = (min(j, a.ns - 1) - s0) * a.nr * a.nt + vz;
-                izoff[i] = min(j, a.ns - 1) * a.nr + vz;
+                soff[i] = (min(js, a.ns - 1) - s0) * a.nr * a.nt + vz;
+                izoff[i] = min(js, a.ns - 1) * a.nr + vz;
             }
 
-            // the window of source i in round r starts at bs + br(r) - kWin / 2;
-            // lane l stages its sample l (a clamped load, flagged in svok when
-            // the sample lies inside the trace; outside ones become zero when
-            // the round begins, so that the loads are waited for only then);
-            // the aperture gate of the round is loaded with it
+            // the window of source i in round r starts at ws = mean over the
+            // group's tiles of bs + br(r), minus kWin / 2; lane l stages
+            // sample ws + l (a clamped load; svok: the pair of position l lies
+            // in [0, nt - 2] and has a neighbour), the aperture gate of the
+            // round is loaded with it
             TD sv[K][JW];
             bool svok[JW];
-            int izn[JW] = {};
+            int wsn[JW], izn[JW] = {};
 #define KX_STAGE(R, BR)                                                                        \
     _Pragma("unroll") for (int i = 0; i < JW; ++i)                                              \
     {                                                                                          \
-        const int e = bs[i] + (BR) - kWin / 2 + lane;                                          \
+        int sum = 0;                                                                           \
+        _Pragma("unroll") for (int j = 0; j < NP; ++j) sum += bs[i][j] + (BR)[j];              \
+        wsn[i] = (sum >> Log2<NP>::v) - kWin / 2;                                              \
+        const int e = wsn[i] + lane;                                                           \
         const int ec = min(max(e, 0), a.nt - 1);                                               \
         _Pragma("unroll") for (int k = 0; k < K; ++k)                                         \
             sv[k][i] = dblk[k * dstride + (unsigned)(soff[i] + (R) * a.nt + ec)];              \
-        svok[i] = (unsigned)e < (unsigned)a.nt;                                                \
+        svok[i] = (unsigned)e < ntm1 && lane < kWin - 1;                                       \
         if constexpr (DYN)                                                                     \
             izn[i] = a.izmin[(unsigned)(izoff[i] + (R))];                                      \
     }
             // receiver side of the round ahead; receiver bases two rounds
-            // ahead (the window addresses of the round ahead depend on them)
-            const size_t t0r = (size_t)t * a.nr;
-            TT orr_c = offr[t0r * kWave + lane];
-            int br_c = a.base_r[t0r + vz];
-            int br_n = a.nr > 1 ? a.base_r[t0r + 1 + vz] : 0;
-            TD pr_c = TD(0), qr_c = TD(0);
-            if constexpr (DYN) {
-                pr_c = wpr[t0r * kWave + lane];
-                qr_c = wqr[t0r * kWave + lane];
+            // ahead (the window positions of the round ahead depend on them)
+            TT orr_c[NP];
+            int br_c[NP], br_n[NP];
+            TD pr_c[NP], qr_c[NP];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const size_t ro = (size_t)t[j] * a.nr;
+                orr_c[j] = offr[ro * kWave + lane];
+                br_c[j] = __builtin_amdgcn_readfirstlane(a.base_r[ro + vz]);
+                br_n[j] = a.nr > 1 ? a.base_r[ro + 1 + vz] : 0;
+                pr_c[j] = qr_c[j] = TD(0);
+                if constexpr (DYN) {
+                    pr_c[j] = wpr[ro * kWave + lane];
+                    qr_c[j] = wqr[ro * kWave + lane];
+                }
             }
             KX_STAGE(0, br_c)
             for (int r = 0; r < a.nr; ++r) {
-                // this round's samples into the wave's LDS region
-                int izr[JW];
+                // this round's pairs into the wave's LDS region
+                int ws[JW], izr[JW];
 #pragma unroll
                 for (int i = 0; i < JW; ++i) {
 #pragma unroll
-                    for (int k = 0; k < K; ++k)
-                        w[k][i][lane] = svok[i] ? sv[k][i] : TD(0);
+                    for (int k = 0; k < K; ++k) {
+                        const TD nx = lane_next(sv[k][i]);
+                        TP q;
+                        q.v[0] = svok[i] ? sv[k][i] : TD(0);
+                        q.v[1] = svok[i] ? nx - sv[k][i] : TD(0);
+                        w[k][i][lane] = q;
+                    }
+                    ws[i] = wsn[i];
                     izr[i] = izn[i];
                 }
-                const TC orr = (TC)ld_off(&orr_c);
-                const int br = br_c;
-                const TD pr = pr_c, qr = qr_c;
+                TC orr[NP];
+                int br[NP];
+                TD pr[NP], qr[NP];
+#pragma unroll
+                for (int j = 0; j < NP; ++j) {
+                    orr[j] = (TC)ld_off(&orr_c[j]);
+                    br[j] = br_c[j];
+                    pr[j] = pr_c[j];
+                    qr[j] = qr_c[j];
+                }
                 // next round's receiver side and samples, in flight meanwhile
                 if (r + 1 < a.nr) {
-                    const size_t ro = t0r + r + 1;
-                    orr_c = offr[ro * kWave + lane];
-                    if constexpr (DYN) {
-                        pr_c = wpr[ro * kWave + lane];
-                        qr_c = wqr[ro * kWave + lane];
+#pragma unroll
+                    for (int j = 0; j < NP; ++j) {
+                        const size_t ro = (size_t)t[j] * a.nr + r + 1;
+                        orr_c[j] = offr[ro * kWave + lane];
+                        if constexpr (DYN) {
+                            pr_c[j] = wpr[ro * kWave + lane];
+                            qr_c[j] = wqr[ro * kWave + lane];
+                        }
+                        br_c[j] = __builtin_amdgcn_readfirstlane(br_n[j]);
                     }
-                    br_c = br_n;
                     KX_STAGE(r + 1, br_c)
-                    if (r + 2 < a.nr)
-                        br_n = a.base_r[ro + 1 + vz];
+                    if (r + 2 < a.nr) {
+#pragma unroll
+                        for (int j = 0; j < NP; ++j)
+                            br_n[j] = a.base_r[(size_t)t[j] * a.nr + r + 2 + vz];
+                    }
                 }
-                // Each lane reads its two samples from the staged windows.  A
-                // sample outside its window (rare: the tile's own offsets span
-                // a few tens of samples) is marked, and the wave fixes the
-                // round's sum afterwards from global memory, so that the
-                // common path has no per-source branch.
+                // Each lane reads one pair per (source, tile) from the staged
+                // windows; a sample outside its window reads the zero pair
+                // and is marked, and the wave adds it afterwards from global
+                // memory, so that the common path has no per-sample branch.
                 unsigned miss = 0;
 #pragma unroll
                 for (int i = 0; i < JW; ++i) {
-                    const TC T = os[i] + orr;
-                    const TC fl = floor(T);
-                    TC f = T - fl;
-                    int it = (int)fl + bs[i] + br;
-                    if constexpr (NEG) {
-                        // the reference truncates toward zero: a time in
-                        // (-1, 0) reads sample 0 with a negative weight
-                        if (it == -1 && f > TC(0)) {
-                            it = 0;
-                            f -= TC(1);
-                        }
-                    }
-                    const int kk = it - (bs[i] + br - kWin / 2);
-                    const int kc = min(max(kk, 0), kWin - 2);
-                    const bool in_trace = (unsigned)it < ntm1;
-                    miss |= (in_trace && kk != kc) ? (1u << i) : 0u;
-                    TD wgt = TD(1);
-                    if constexpr (DYN) {
-                        wgt = ps[i] * pr + qs[i] * qr;
-                        if (p.iz < izr[i])
-                            wgt = TD(0);
-                    }
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const TD q0 = w[k][i][kc];
-                        const TD q1 = w[k][i][kc + 1];
-                        TD v = fma((TD)f, q1 - q0, q0);
-                        if constexpr (DYN)
-                            v *= wgt;
-                        acc[k] += in_trace ? v : TD(0);
-                    }
-                }
-                if (__ballot(miss != 0) != 0) {
-                    // replace the window values of the marked sources by the
-                    // samples themselves
-#pragma unroll
-                    for (int i = 0; i < JW; ++i) {
-                        if ((miss >> i) & 1u) {
-                            const TC T = os[i] + orr;
+                    for (int j = 0; j < NP; ++j) {
+                        const int db = bs[i][j] + br[j] - ws[i];  // wave-uniform
+                        const TC T = os[i][j] + orr[j];
+                        TC f;
+                        int kk;
+                        if constexpr (!NEG && sizeof(TC) == 4) {
+                            f = __builtin_amdgcn_fractf(T);
+                            kk = floor_int(T) + db;
+                        } else {
                             const TC fl = floor(T);
-                            TC f = T - fl;
-                            int it = (int)fl + bs[i] + br;
+                            f = T - fl;
+                            kk = (int)fl + db;
                             if constexpr (NEG) {
-                                if (it == -1 && f > TC(0)) {
-                                    it = 0;
+                                // the reference truncates toward zero: a time
+                                // in (-1, 0) reads sample 0 with a negative
+                                // weight
+                                if (kk + ws[i] == -1 && f > TC(0)) {
+                                    kk += 1;
                                     f -= TC(1);
                                 }
                             }
-                            const int kk = it - (bs[i] + br - kWin / 2);
-                            const int kc = min(max(kk, 0), kWin - 2);
-                            TD wgt = TD(1);
-                            if constexpr (DYN) {
-                                wgt = ps[i] * pr + qs[i] * qr;
-                                if (p.iz < izr[i])
-                                    wgt = TD(0);
-                            }
+                        }
+                        const bool inw = (unsigned)kk <= (unsigned)(kWin - 2);
+                        miss |= inw ? 0u : (1u << (i * NP + j));
+                        const int kc = inw ? kk : kWin - 1;
+                        TD wgt = TD(1);
+                        if constexpr (DYN) {
+                            wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
+                            if (pt[j].iz < izr[i])
+                                wgt = TD(0);
+                        }
 #pragma unroll
-                            for (int k = 0; k < K; ++k) {
-                                const TD *q = dblk + k * dstride + (soff[i] + r * a.nt + it);
-                                const TD g0 = q[0], g1 = q[1];
-                                const TD l0 = w[k][i][kc], l1 = w[k][i][kc + 1];
-                                TD dv = fma((TD)f, g1 - g0, g0) - fma((TD)f, l1 - l0, l0);
-                                if constexpr (DYN)
-                                    dv *= wgt;
-                                acc[k] += dv;
+                        for (int k = 0; k < K; ++k) {
+                            const TP q = w[k][i][kc];
+                            TD v = fma((TD)f, q.v[1], q.v[0]);
+                            if constexpr (DYN)
+                                v *= wgt;
+                            acc[k][j] += v;
+                        }
+                    }
+                }
+                if (__ballot(miss != 0) != 0) {
+                    // the marked samples, from global memory
+#pragma unroll
+                    for (int i = 0; i < JW; ++i) {
+#pragma unroll
+                        for (int j = 0; j < NP; ++j) {
+                            if ((miss >> (i * NP + j)) & 1u) {
+                                const TC T = os[i][j] + orr[j];
+                                const TC fl = floor(T);
+                                TC f = T - fl;
+                                int it = (int)fl + bs[i][j] + br[j];
+                                if constexpr (NEG) {
+                                    if (it == -1 && f > TC(0)) {
+                                        it = 0;
+                                        f -= TC(1);
+                                    }
+                                }
+                                if ((unsigned)it < ntm1) {
+                                    TD wgt = TD(1);
+                                    if constexpr (DYN) {
+                                        wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
+                                        if (pt[j].iz < izr[i])
+                                            wgt = TD(0);
+                                    }
+#pragma unroll
+                                    for (int k = 0; k < K; ++k) {
+                                        const TD *q = dblk + k * dstride + (soff[i] + r * a.nt + it);
+                                        const TD g0 = q[0], g1 = q[1];
+                                        TD v = fma((TD)f, g1 - g0, g0);
+                                        if constexpr (DYN)
+                                            v *= wgt;
+                                        acc[k][j] += v;
+                                    }
+                                }
                             }
                         }
                     }
@@ -975,28 +1140,37 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
         }
     }
     if constexpr (P > 1) {
-        // the partial sums of a tile's points, added in part order
+        // the partial sums of a group's points, added in part order
         __syncthreads();
-        TD *red = &win[0][0][0][0];
+        TD *red = reinterpret_cast<TD *>(&win[0][0][0][0]);
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            red[k * kGatherBlock + threadIdx.x] = acc[k];
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+                red[(k * NP + j) * kGatherBlock + threadIdx.x] = acc[k][j];
         __syncthreads();
         if (part == 0) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                acc[k] = TD(0);
+            for (int k = 0; k < K; ++k)
 #pragma unroll
-                for (int q = 0; q < P; ++q)
-                    acc[k] += red[k * kGatherBlock + (q * NT + wave) * kWave + lane];
-            }
+                for (int j = 0; j < NP; ++j) {
+                    acc[k][j] = TD(0);
+#pragma unroll
+                    for (int q = 0; q < P; ++q)
+                        acc[k][j] += red[(k * NP + j) * kGatherBlock + (q * NG + wave) * kWave + lane];
+                }
         }
     }
-    if (part == 0 && active && p.valid) {
-        const int64_t pi = point_index(a, p);
+    if (part == 0 && active) {
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            m[k * mstride + pi] = acc[k];
+        for (int j = 0; j < NP; ++j) {
+            if (tok[j] && pt[j].valid) {
+                const int64_t pi = point_index(a, pt[j]);
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    m[k * mstride + pi] = acc[k][j];
+            }
+        }
     }
     if constexpr (P > 1)
         __syncthreads();  // the partial sums are read before a next item stages

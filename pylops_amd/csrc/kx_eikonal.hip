@@ -280,7 +280,7 @@ extern "C" int kx_eikonal(int32_t ndims, int32_t nx, int32_t ny, int32_t nz, dou
         return efail((int)e, "eikonal init: %s", hipGetErrorString(e));
     const int ndiag = (nx - 1) + (ny - 1) + (nz - 1) + 1;
     const int ndir = ndims == 3 ? 8 : 4;
-    int rounds = 0;
+    int rounds = 0, converged = 1;
     // first-order sweeps to convergence, then (order 2) second-order ones
     for (int phase = 1; phase <= order; ++phase) {
         for (int it = 0; it < max_iter; ++it) {
@@ -312,9 +312,13 @@ extern "C" int kx_eikonal(int32_t ndims, int32_t nx, int32_t ny, int32_t nz, dou
                 return efail((int)e, "eikonal: %s", hipGetErrorString(e));
             if (!h)
                 break;
+            if (it == max_iter - 1)
+                converged = 0;  // values still changed in the last round
         }
     }
-    if (iters)
-        *iters = rounds;
+    if (iters) {
+        iters[0] = rounds;
+        iters[1] = converged;
+    }
     return KX_OK;
 }

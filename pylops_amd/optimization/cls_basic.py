@@ -102,6 +102,23 @@ def _replay(graph, n, sc, active_index, device):
 _capture_streams = {}
 
 
+def _recount(Op, counts, iters):
+    """Set the operator's product counters to their values at ``counts`` plus
+    one matvec and one rmatvec per iteration run: a replayed graph never
+    passes through the counting wrappers, and iterations enqueued after
+    convergence are skipped on the device (CGLS.step and LSQR.step of
+    CB:510-564, 987-1136 each apply Op and Op^H once)."""
+    if counts is not None:
+        Op.matvec_count = counts[0] + iters
+        Op.rmatvec_count = counts[1] + iters
+
+
+def _counts(Op):
+    if hasattr(Op, "matvec_count") and hasattr(Op, "rmatvec_count"):
+        return Op.matvec_count, Op.rmatvec_count
+    return None
+
+
 def _capture(body, device):
     """Capture ``body`` in a HIP graph on a side stream (one per device,
     default priority, reused).  torch's ``graph`` context would first empty
@@ -258,6 +275,7 @@ class CGLS(Solver):
         niter = self.niter if niter is None else niter
         if niter is None:
             raise ValueError("niter must not be None")
+        counts, it0 = _counts(self.Op), self.iiter
         self._grow(niter + 1)
         self._d.sc[_lib.KX_CG_NITER] = float(niter)
         self._d.sc[_lib.KX_CG_ACTIVE] = (
@@ -286,6 +304,7 @@ class CGLS(Solver):
                         break
                 x = self.step(x)
             self._sync_state()
+        _recount(self.Op, counts, self.iiter - it0)
         return x
 
     def finalize(self, show=False):
@@ -498,6 +517,7 @@ class LSQR(Solver):
         """CB:1138-1186.  ``graph=True``: first iteration eager, the next one
         captured in a HIP graph and replayed."""
         niter = self.niter if niter is None else niter
+        counts, it0 = _counts(self.Op), self.iiter
         self._grow(niter + 1)
         D = self._d
         D.sc[_lib.KX_LS_NITER] = float(niter)
@@ -530,6 +550,7 @@ class LSQR(Solver):
                         break
                 x = self.step(x)
             self._sync_state()
+        _recount(self.Op, counts, self.iiter - it0)
         return x
 
     def finalize(self, show=False):

@@ -101,26 +101,32 @@ def eikonal(pts, vel, origin, dsamp, dims, device, unit=False, max_iter=20, orde
     srcd = torch.as_tensor(np.ascontiguousarray(src, dtype=np.int32), device=device)
     out = torch.empty((ni, n), dtype=torch.float64, device=device)
     flag = torch.zeros(1, dtype=torch.int32, device=device)
-    iters = ctypes.c_int32(0)
+    iters = (ctypes.c_int32 * 2)(0, 0)
     L = _lib.load()
     rc = L.kx_eikonal(ndims, nx, ny, nz, hx, hy, hz, _lib.ptr(slow.contiguous()), n,
                       _lib.ptr(srcd), _lib.ptr(out), max_iter, order, _lib.ptr(flag),
-                      ctypes.byref(iters), _lib.stream_ptr(device))
+                      iters, _lib.stream_ptr(device))
     _lib.check(rc, "kx_eikonal")
+    if not iters[1]:
+        # an unconverged sweep leaves upper bounds, not first arrivals
+        raise RuntimeError(
+            f"eikonal traveltimes did not converge in {max_iter} sweep rounds per phase "
+            f"({iters[0]} rounds run); pass a larger eikonal_iter")
     return out
 
 
-def eikonal_tables(srcs, recs, vel, origin, dsamp, dims, device, distances):
+def eikonal_tables(srcs, recs, vel, origin, dsamp, dims, device, distances, max_iter=20):
     """(trav_srcs, trav_recs, dist_srcs, dist_recs, grad_srcs, grad_recs) of
     the eikonal mode (K:609-657): first-arrival times from each source and
     receiver cell, the unit-speed distances when ``distances`` (dynamic
-    amplitudes need them), and the traveltime gradients."""
-    ts = eikonal(srcs, vel, origin, dsamp, dims, device)
-    tr = eikonal(recs, vel, origin, dsamp, dims, device)
+    amplitudes need them), and the traveltime gradients; ``max_iter`` sweep
+    rounds per phase at most (RuntimeError if they do not converge)."""
+    ts = eikonal(srcs, vel, origin, dsamp, dims, device, max_iter=max_iter)
+    tr = eikonal(recs, vel, origin, dsamp, dims, device, max_iter=max_iter)
     ds = dr = None
     if distances:
-        ds = eikonal(srcs, None, origin, dsamp, dims, device, unit=True)
-        dr = eikonal(recs, None, origin, dsamp, dims, device, unit=True)
+        ds = eikonal(srcs, None, origin, dsamp, dims, device, unit=True, max_iter=max_iter)
+        dr = eikonal(recs, None, origin, dsamp, dims, device, unit=True, max_iter=max_iter)
     return ts, tr, ds, dr, gradient(ts, dims, dsamp), gradient(tr, dims, dsamp)
 
 

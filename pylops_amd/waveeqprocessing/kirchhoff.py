@@ -32,7 +32,7 @@ _ENGINES = ("hip", "cuda", "numba", "numpy")
 # small attributes whose reassignment invalidates the device tables
 _PLAIN_ATTRS = ("vel", "aperture", "angleaperture", "six", "rix", "tile", "tabledtype")
 # launch-shape attributes: copied into the plan, the tables stay
-_LAUNCH_ATTRS = ("rblock", "sorder", "gsplit", "sside")
+_LAUNCH_ATTRS = ("rblock", "sorder", "gsplit", "sside", "gtiles")
 
 
 class _Table:
@@ -77,9 +77,14 @@ class Kirchhoff(LinearOperator):
     device : torch device of the tables (default: current HIP device).
     rblock : receivers (or sources, see ``sside``) per forward workgroup
         (0 = automatic).
-    sorder : forward lane order, ``KX_ORDER_NATURAL`` (1) or
-        ``KX_ORDER_ROTATED`` (2); 0 = automatic.
-    gsplit : adjoint waves per tile, 1, 2 or 4; 0 = automatic.
+    sorder : forward lane order, ``KX_ORDER_NATURAL`` (1),
+        ``KX_ORDER_ROTATED`` (2) or ``KX_ORDER_GROUPED`` (3); 0 = automatic.
+    gsplit : adjoint waves per tile group, 1, 2 or 4; 0 = automatic.
+    gtiles : neighbouring image tiles per adjoint wave, sharing its staged
+        trace windows, 1, 2 or 4; 0 = automatic.
+    eikonal_iter : sweep rounds per phase of the eikonal solver at most
+        (``mode="eikonal"``); tables that do not converge within them raise
+        ``RuntimeError``.
     sside : what a forward workgroup blocks, ``KX_SIDE_RECEIVERS`` (1: one
         source, a block of receivers) or ``KX_SIDE_SOURCES`` (2: one
         receiver, a block of sources; float32 operators); 0 = automatic.
@@ -101,7 +106,8 @@ class Kirchhoff(LinearOperator):
                  mode="eikonal", wavfilter=False, dynamic=False, trav=None, amp=None,
                  aperture=None, angleaperture=90.0, snell=None, engine="hip",
                  dtype="float64", name="K", *, tabledtype=None, tile=None,
-                 device=None, rblock=0, sorder=0, gsplit=0, sside=0):
+                 device=None, rblock=0, sorder=0, gsplit=0, sside=0, gtiles=0,
+                 eikonal_iter=20):
         object.__setattr__(self, "_dirty", True)
         object.__setattr__(self, "_tab", {})
         object.__setattr__(self, "_tab_host", {})
@@ -131,7 +137,8 @@ class Kirchhoff(LinearOperator):
                 # the reference solves with scikit-fmm (K:609-638); here the
                 # device fast-sweeping solver (kx_eikonal)
                 ts, tr, ds, dr, gs, gr = T.eikonal_tables(srcs, recs, vel, origin, dsamp, dims,
-                                                          dev, self.dynamic)
+                                                          dev, self.dynamic,
+                                                          max_iter=eikonal_iter)
             else:
                 if not isinstance(vel, (float, int)):
                     raise ValueError("vel must be scalar for mode=analytical")
@@ -177,6 +184,12 @@ class Kirchhoff(LinearOperator):
             self.wav = wav
         self.cop = Convolve1D((ns * nr, self.nt), h=np.asarray(self.wav), offset=wavcenter,
                               axis=1, dtype=dtype)
+        # the reference's forward convolves its traces with scipy, which
+        # returns the result type of traces and wavelet: a float32 operator
+        # with a float64 wavelet (or wavfilter=True, whose reshaped wavelet
+        # is float64) returns float64 data (C1D:99-106)
+        object.__setattr__(self, "_fwd_dtype",
+                           np.result_type(np.dtype(dtype), np.asarray(self.wav).dtype))
         self.aperturetap = taper(41, 20, "hanning")[20:]
 
         if aperture is not None and self.ndims == 3:
@@ -219,6 +232,7 @@ class Kirchhoff(LinearOperator):
         self.sorder = int(sorder)
         self.sside = int(sside)
         self.gsplit = int(gsplit)
+        self.gtiles = int(gtiles)
         self.engine = engine
         if engine == "cuda" and not self.travsrcrec:
             raise NotImplementedError("engine='cuda' not implemented for traveltimes "
@@ -367,6 +381,12 @@ class Kirchhoff(LinearOperator):
                                     device=dev)
         keep["hstar"] = torch.as_tensor(np.asarray(self.cop.hstar, dtype=np.float64),
                                         dtype=tdt, device=dev)
+        if self._fwd_dtype != self.dtype:
+            # float64 wavelet of a float32 operator: float64 convolutions
+            keep["h64"] = torch.as_tensor(np.asarray(self.cop.h, dtype=np.float64),
+                                          device=dev)
+            keep["hstar64"] = torch.as_tensor(np.asarray(self.cop.hstar, dtype=np.float64),
+                                              device=dev)
         plan.dtype = wfmt
         plan.tab_fmt = fmt
         plan.dynamic = int(self.dynamic)
@@ -376,6 +396,7 @@ class Kirchhoff(LinearOperator):
         plan.sorder = self.sorder
         plan.sside = self.sside
         plan.gsplit = self.gsplit
+        plan.gtiles = self.gtiles
         plan.nh = keep["h"].numel()
         for k, v in keep.items():
             setattr(plan, k, v.data_ptr())
@@ -427,12 +448,31 @@ class Kirchhoff(LinearOperator):
     # ------------------------------------------------------------------
     # products (K:1048-1143)
     # ------------------------------------------------------------------
+    def _conv64(self, x, h):
+        """float64 "same" convolution of every trace of x with h (device)."""
+        out = torch.empty(self.nsnr * self.nt, dtype=torch.float64, device=self.device)
+        rc = _lib.load().kx_conv(_lib.ptr(x), _lib.ptr(out), self.nsnr, self.nt, _lib.ptr(h),
+                                 h.numel(), _lib.KX_F64, _lib.stream_ptr(self.device))
+        _lib.check(rc, "kx_conv")
+        return out
+
     @reshaped
     def _matvec(self, x):
+        if self._fwd_dtype != self.dtype:
+            # float32 traces, convolved in float64 with the float64 wavelet
+            # (the reference's scipy upcast, C1D:99-106): float64 data
+            traces = self.spread(to_device(x.reshape(-1), self.dtype, self.device))
+            return like_input(self._conv64(traces.double(), self._dev_tables["h64"]), x)
         return self._launch(_lib.load().kx_forward, x, self.nsnr * self.nt, "kx_forward")
 
     @reshaped
     def _rmatvec(self, x):
+        if self._fwd_dtype != self.dtype:
+            # correlation in float64 (scipy's upcast), then the float32 gather
+            self.plan  # noqa: B018  (rebuilds stale device tables)
+            xd = to_device(x.reshape(-1), np.float64, self.device)
+            dc = self._conv64(xd, self._dev_tables["hstar64"])
+            return like_input(self.gather(dc.to(torch.float32)), x)
         return self._launch(_lib.load().kx_adjoint, x, self.ni, "kx_adjoint")
 
     # several columns at once: the reference's matmat applies matvec column
@@ -490,12 +530,12 @@ class Kirchhoff(LinearOperator):
         return like_input(out.T, X)
 
     def _matmat(self, X):
-        if X.shape[1] == 1:
+        if X.shape[1] == 1 or self._fwd_dtype != self.dtype:
             return super()._matmat(X)
         return self._multi(X, False)
 
     def _rmatmat(self, X):
-        if X.shape[1] == 1:
+        if X.shape[1] == 1 or self._fwd_dtype != self.dtype:
             return super()._rmatmat(X)
         return self._multi(X, True)
 

@@ -3,7 +3,7 @@
 golden vectors (tests/golden/make_golden.py)."""
 import numpy as np
 
-from oracle.oracle import ricker
+from pylops_amd.utils.wavelets import ricker
 
 PAR = dict(ny=3, nx=12, nz=20, nt=50, dy=3, dx=1, dz=2, dt=0.004,
            nsy=4, nry=3, nsx=6, nrx=2)

@@ -19,9 +19,10 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda"
 
 
-def _device_tables(slow, src, h, order=2):
+def _device_tables(slow, src, h, order=2, max_iter=20, converged=None):
     """kx_eikonal through the C-ABI for slow[ny, nx, nz] and source cells
-    (ix, iy, iz); returns (T[n, ny, nx, nz], rounds)."""
+    (ix, iy, iz); returns (T[n, ny, nx, nz], rounds), and checks the
+    convergence flag against ``converged`` (True by default)."""
     import ctypes
     from pylops_amd import _lib
     ny, nx, nz = slow.shape
@@ -31,13 +32,14 @@ def _device_tables(slow, src, h, order=2):
     sd = torch.as_tensor(np.asarray(src, dtype=np.int32).reshape(-1, 3), device=DEV)
     out = torch.empty((ny * nx * nz, n), dtype=torch.float64, device=DEV)
     flag = torch.zeros(1, dtype=torch.int32, device=DEV)
-    it = ctypes.c_int32(0)
+    it = (ctypes.c_int32 * 2)(0, 0)
     L = _lib.load()
     rc = L.kx_eikonal(ndims, nx, ny, nz, h[0], h[1], h[2], _lib.ptr(s), n, _lib.ptr(sd),
-                      _lib.ptr(out), 20, order, _lib.ptr(flag), ctypes.byref(it),
+                      _lib.ptr(out), max_iter, order, _lib.ptr(flag), it,
                       _lib.stream_ptr(torch.device(DEV)))
     _lib.check(rc, "kx_eikonal")
-    return out.cpu().numpy().T.reshape(n, ny, nx, nz), it.value
+    assert bool(it[1]) == (True if converged is None else converged), tuple(it)
+    return out.cpu().numpy().T.reshape(n, ny, nx, nz), it[0]
 
 
 @pytest.mark.parametrize("order", [1, 2])
@@ -162,3 +164,23 @@ def test_source_outside_grid_raises():
     # the C-ABI checks the cells itself (KX_ESHAPE)
     with pytest.raises(ValueError, match="outside"):
         _device_tables(np.ones((1, 5, 4)), [(5, 0, 0)], (1.0, 1.0, 1.0))
+
+
+def test_unconverged_sweeps_are_reported():
+    """A serpentine fast corridor (tests.test_oracle.serpentine) needs more
+    than 20 first-order sweep rounds: the device reports the unconverged
+    tables (iters[1] = 0; the table builder raises), and with enough rounds
+    it converges in the oracle's number of rounds to the oracle's times."""
+    from pylops_amd.waveeqprocessing import _tables as T
+    from tests.test_oracle import serpentine
+    slow, src, h = serpentine()
+    _device_tables(slow, src, h, order=1, max_iter=20, converged=False)
+    got, it_dev = _device_tables(slow, src, h, order=1, max_iter=200)
+    want, it_ora = O.eikonal_fsm(slow, src, h, max_iter=200, order=1)
+    assert it_dev == it_ora > 20
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+    ny, nx, nz = slow.shape
+    pts = np.array([[src[0][0] * h[0]], [src[0][2] * h[2]]])
+    with pytest.raises(RuntimeError, match="did not converge"):
+        T.eikonal(pts, 1.0 / slow[0], (0.0, 0.0), (h[0], h[2]), (nx, nz), DEV, max_iter=20,
+                  order=1)

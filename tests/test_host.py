@@ -65,7 +65,7 @@ def test_header_constants_match_binding():
     from pylops_amd import _lib
     text = open(HEADER).read()
     for name in ("KX_ABI_VERSION", "KX_AUX_BYTES", "KX_F32", "KX_F64", "KX_F16", "KX_ORDER_AUTO",
-                 "KX_ORDER_NATURAL", "KX_ORDER_ROTATED", "KX_SIDE_AUTO", "KX_SIDE_RECEIVERS",
+                 "KX_ORDER_NATURAL", "KX_ORDER_ROTATED", "KX_ORDER_GROUPED", "KX_SIDE_AUTO", "KX_SIDE_RECEIVERS",
                  "KX_SIDE_SOURCES",
                  "KX_EINVAL", "KX_ESHAPE", "KX_ETYPE", "KX_EABI", "KX_ELDS"):
         m = re.search(rf"#define {name} (-?\d+)", text)
@@ -348,3 +348,25 @@ def test_solver_callback_hooks_wrap_methods():
         ("b", "step_end", 6), ("a", "step_end", 6),
         ("b", "run_end", 5), ("a", "run_end", 5)]
     assert all(sv is s for _, _, sv, _ in seen)
+
+
+@pytest.mark.parametrize("nt", [21, 20])
+def test_ricker(nt):
+    """pytests/test_wavelets.py:58-72 restated: size of the symmetric axis
+    and wavelet (one sample dropped from an even axis) and a unit peak at
+    the centre; plus the closed form (1 - 2a) exp(-a), a = (pi f0 t)^2, and
+    the mirror symmetry."""
+    import warnings
+    from pylops_amd.utils.wavelets import ricker
+    t = np.arange(nt) * 0.004
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        wav, twav, wcenter = ricker(t, f0=20)
+    n = (nt - 1 if nt % 2 == 0 else nt) * 2 - 1
+    assert twav.size == n
+    assert wav.shape[0] == n
+    assert wav[wcenter] == 1
+    assert wcenter == n // 2 and twav[wcenter] == 0
+    a = (np.pi * 20 * twav) ** 2
+    np.testing.assert_allclose(wav, (1 - 2 * a) * np.exp(-a), rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(wav, wav[::-1])

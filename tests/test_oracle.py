@@ -408,3 +408,35 @@ def test_shrinkage_analysis_form_with_orthonormal_s(fast):
     xa, _, _ = O.shrinkage(O.DenseOp(A), y, alpha, niter=15, eps=0.3, tol=0, fast=fast, S=S)
     c, _, _ = O.shrinkage(O.DenseOp(A @ S), y, alpha, niter=15, eps=0.3, tol=0, fast=fast)
     np.testing.assert_allclose(xa, S @ c, rtol=1e-10, atol=1e-12)
+
+
+def serpentine(nlegs=48, width=9):
+    """A 2D medium whose first arrivals follow a serpentine corridor: rows
+    of slowness 1e-3 s/m at every second depth, joined at alternate ends,
+    walls of slowness 1 s/m (h = 1 m), the source at the corridor's start.
+    Fast sweeping advances about two corridor legs per round of its four
+    sweep directions, so the corridor needs more than 20 rounds."""
+    nz = 2 * nlegs - 1
+    slow = np.ones((1, width, nz))
+    for leg in range(nlegs):
+        slow[0, :, 2 * leg] = 1e-3
+        if leg + 1 < nlegs:
+            slow[0, width - 1 if leg % 2 == 0 else 0, 2 * leg + 1] = 1e-3
+    return slow, [(0, 0, 0)], (1.0, 1.0, 1.0)
+
+
+def test_eikonal_needs_more_rounds_in_a_serpentine():
+    """The oracle's fast sweeping on the serpentine corridor: 20 first-order
+    rounds leave the far end of the corridor unconverged (far above its
+    converged time), and the converged times follow the corridor (the
+    arrival at its end is the corridor length times its slowness, to the
+    accuracy of the first-order scheme along a straight corridor)."""
+    slow, src, h = serpentine()
+    T20, r20 = O.eikonal_fsm(slow, src, h, max_iter=20, order=1)
+    T, r = O.eikonal_fsm(slow, src, h, max_iter=200, order=1)
+    assert r20 == 20 and 20 < r < 200
+    nz = slow.shape[2]
+    end = T[0, 0, 0 if (nz // 2) % 2 else slow.shape[1] - 1, nz - 1]
+    path = (slow.shape[1] - 1) * (nz + 1) // 2 + (nz - 1)   # cells along the corridor
+    assert abs(end - path * 1e-3) < 0.05 * path * 1e-3
+    assert T20[0, 0, :, nz - 1].min() > 2 * T[0, 0, :, nz - 1].max()

@@ -50,10 +50,14 @@ def test_cgls_lsqr_vs_golden():
     y = g["dobs"]
     x, istop, iiter, r1, r2, cost = cgls(Op, y, niter=10, tol=0.0)
     assert (istop, iiter) == (int(g["cgls_istop"]), int(g["cgls_iiter"]))
-    np.testing.assert_allclose(x, g["cgls_x"].reshape(x.shape), rtol=1e-7, atol=1e-9)
+    # solutions come back with the reference's shapes: Op.dims, or flat
+    # when x0 was given flat (decorators.py:38-70)
+    assert x.shape == g["cgls_x"].shape == tuple(Op.dims)
+    np.testing.assert_allclose(x, g["cgls_x"], rtol=1e-7, atol=1e-9)
     np.testing.assert_allclose([r1, r2], [g["cgls_r1"], g["cgls_r2"]], rtol=1e-7)
     np.testing.assert_allclose(cost, g["cgls_cost"], rtol=1e-8)
     x, istop, iiter, r1, r2, cost = cgls(Op, y, x0=g["x0"], niter=8, damp=0.3, tol=0.0)
+    assert x.shape == g["cglsd_x"].shape
     np.testing.assert_allclose(x, g["cglsd_x"], rtol=1e-7, atol=1e-9)
     np.testing.assert_allclose([r1, r2], [g["cglsd_r1"], g["cglsd_r2"]], rtol=1e-7)
     np.testing.assert_allclose(cost, g["cglsd_cost"], rtol=1e-8)
@@ -61,6 +65,7 @@ def test_cgls_lsqr_vs_golden():
              "cost"]
     for pre, kw in (("lsqr_", dict(niter=10)), ("lsqrd_", dict(x0=g["x0"], damp=0.2, niter=7))):
         res = lsqr(Op, y, calc_var=True, **kw)
+        assert res[0].shape == g[pre + "x"].shape
         for k, v in zip(names, res):
             np.testing.assert_allclose(np.ravel(v), np.ravel(g[pre + k]), rtol=1e-6,
                                        atol=1e-9, err_msg=pre + k)
@@ -83,6 +88,31 @@ def test_cgls_tol_early_stop_matches_oracle():
     assert res[2] == ref[2] and res[1] == ref[1]
     np.testing.assert_allclose(res[0].ravel(), ref[0], rtol=1e-6, atol=1e-9)
     np.testing.assert_allclose(res[5], ref[5], rtol=1e-7)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_solver_product_counts(graph):
+    """The operator's counters after a solve count the reference's products:
+    CGLS setup Op^H and Op once each, LSQR setup Op^H once, then one Op and
+    one Op^H per iteration run -- also when the iterations are replayed
+    from a HIP graph and when an early stop skips iterations that were
+    already enqueued."""
+    from pylops_amd.optimization.basic import cgls, lsqr
+    Op = _kop()
+    orc = O.KirchhoffOracle(G.z, G.x, G.t, G.s2d, G.r2d, G.v0, G.wav, G.wavc,
+                            mode="analytic", dynamic=True)
+    y = np.load(os.path.join(G.GOLDEN, "solvers_dyn2d_f64.npz"))["dobs"]
+    k4 = O.cgls(orc, y, niter=4, tol=0.0)[3]
+    k5 = O.cgls(orc, y, niter=5, tol=0.0)[3]
+    for tol, want in ((float(np.sqrt(k4 * k5)), 5), (0.0, 30)):
+        Op.matvec_count = Op.rmatvec_count = 0
+        res = cgls(Op, y, niter=30, tol=tol, graph=graph)
+        assert res[2] == want
+        assert (Op.matvec_count, Op.rmatvec_count) == (1 + want, 1 + want)
+    Op.matvec_count = Op.rmatvec_count = 0
+    res = lsqr(Op, y, niter=12, graph=graph)
+    assert res[2] == 12
+    assert (Op.matvec_count, Op.rmatvec_count) == (12, 13)
 
 
 @pytest.mark.parametrize("shape", [(11, 11), (31, 11)])
