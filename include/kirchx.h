@@ -48,13 +48,11 @@ extern "C" {
 #define KX_AUX_BYTES 4096
 
 /* lane order of the forward (kx_plan.sorder): at step r of a tile the lane
- * of point p treats receiver r (natural), (p + r) mod rblock (rotated) or
- * (p / 16 + r) mod rblock (grouped) of its workgroup's block; every order
- * gives bitwise the same traces */
+ * of point p treats receiver r (natural) or (p + r) mod rblock (rotated) of
+ * its workgroup's block; both orders give bitwise the same traces */
 #define KX_ORDER_AUTO 0
 #define KX_ORDER_NATURAL 1
 #define KX_ORDER_ROTATED 2
-#define KX_ORDER_GROUPED 3
 
 /* what a forward workgroup blocks (kx_plan.sside): one source and a block of
  * rblock receivers, or one receiver and a block of rblock sources (float32
@@ -79,8 +77,9 @@ extern "C" {
 
 /* Operator plan.  All table pointers are device pointers in the tiled
  * layout produced by kx_encode_table / kx_tile_table:
- *   offsets  off[ntiles][n][64]   element type tab_fmt (f32, f16 or f64),
- *            traveltime in samples minus base,
+ *   offsets  off[ntiles][n][64]   element type tab_fmt (f32, f16 or f64;
+ *            with KX_F16 the source side off_s is f32), traveltime in
+ *            samples minus base,
  *   bases    base[ntiles][n]      int32, samples,
  *   weights  wp/wq[ntiles][n][64] element type dtype (dynamic only),
  * n = ns for the source side and nr for the receiver side.
@@ -191,7 +190,10 @@ int kx_xcorr_rows(const void *a, int64_t astride, int32_t la, const void *b, int
                   void *stream);
 
 /* Full operator: forward = spread then convolve with h; adjoint =
- * correlate (convolve with hstar) then gather.  work: kx_workspace_bytes. */
+ * correlate (convolve with hstar) then gather.  work: kx_workspace_bytes.
+ * A float32 forward whose workgroups hold their traces as 64-bit sums
+ * convolves them in LDS in the same launch (the data are bitwise those of
+ * kx_spread followed by kx_conv) and leaves work untouched. */
 int kx_forward(const kx_plan *p, const void *m, void *d, void *work, void *stream);
 int kx_adjoint(const kx_plan *p, const void *d, void *m, void *work, void *stream);
 

@@ -1,5 +1,9 @@
-// kirchx.hip -- MI355X (gfx950) Kirchhoff demigration / migration kernels
-// and their C ABI (include/kirchx.h).
+// kirchx.hip -- MI355X (gfx950) Kirchhoff demigration / migration library
+// (include/kirchx.h): error reporting, plan checks, the tiled table layout,
+// the wavelet convolution, the operator composition (kx_forward /
+// kx_adjoint) and the float64 reductions.  The forward launches live in
+// kx_spread.hip, the adjoint launches in kx_gather.hip, the kernel bodies in
+// kx_device.h.
 //
 // Data layout in HBM (all caller-owned):
 //   model  m[ni]              reference order ii = (iy*nx + ix)*nz + iz
@@ -17,10 +21,11 @@
 // streamed through the waves, every (point, trace) pair adds two linearly
 // interpolated samples with LDS atomics (64-bit fixed point for float data),
 // and the traces are written to HBM once.  No global atomics.
-// Adjoint (gather, replaces K:754-781 / K:891-1001): one wavefront owns one
-// image tile; source offsets for a block of sources are held in registers,
-// receiver offsets are streamed coalesced, and each lane gathers its two
-// samples per trace.  Every output element is written once.
+// Adjoint (gather, replaces K:754-781 / K:891-1001): one wavefront owns a
+// group of neighbouring image tiles and a block of sources; receiver offsets
+// are streamed coalesced, each source's trace window is staged in LDS as
+// (sample, difference) pairs, and each point reads one pair per trace.
+// Every output element is written once.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -30,7 +35,7 @@
 #include <cstdint>
 #include <cstdio>
 
-#include "kirchx.h"
+#include "kx_internal.h"
 
 // ---------------------------------------------------------------------------
 // error reporting
@@ -53,17 +58,19 @@ static int kx_fail(int code, const char *fmt, ...)
             return kx_fail((int)e_, "%s: %s", #call, hipGetErrorString(e_));   \
     } while (0)
 
-#define KX_LAUNCHED(name)                                                      \
-    do {                                                                       \
-        hipError_t e_ = hipGetLastError();                                     \
-        if (e_ != hipSuccess)                                                  \
-            return kx_fail((int)e_, "%s launch: %s", name, hipGetErrorString(e_)); \
-    } while (0)
-
-// error hook for the other translation units of the library (not exported)
+// error hooks for the other translation units of the library (not exported)
 extern "C" __attribute__((visibility("hidden"))) int kx_solver_fail(int code, const char *msg)
 {
     return kx_fail(code, "%s", msg);
+}
+
+extern "C" __attribute__((visibility("hidden"))) int kx_failf(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
 }
 
 extern "C" int kx_version(void) { return KX_ABI_VERSION; }
@@ -85,46 +92,6 @@ extern "C" const char *kx_last_error(void) { return g_err; }
 // single-problem kernels
 // ---------------------------------------------------------------------------
 namespace {
-
-template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int RB, bool XS>
-__global__ __launch_bounds__(kSpreadBlock) void spread_kernel(KArgs a, const TD *__restrict__ m,
-                                                              TD *__restrict__ y,
-                                                              const SpreadScale *__restrict__ sc)
-{
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const SpreadScale scale = *sc;
-    spread_item<TD, TT, DYN, NEG, ORD, RB, XS>(a, m, y, scale, L % a.ns, (L / a.ns) * RB);
-}
-
-// K models of a float32 operator; workgroup L (after the XCD renumbering) =
-// source L mod ns, receiver L / ns.  The launch accumulates in fixed point
-// when every model's scale allows it, in float64 otherwise.
-template <typename TT, bool DYN, bool NEG, int K>
-__global__ __launch_bounds__(kSpreadBlock) void spread_models_kernel(
-    KArgs a, const float *__restrict__ m, int64_t ni, float *__restrict__ y, int64_t ystride,
-    const SpreadScale *__restrict__ sc)
-{
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    bool fixed = true;
-#pragma unroll
-    for (int j = 0; j < K; ++j)
-        fixed = fixed && sc[j].fixed;
-    if (fixed)
-        spread_models_body<TT, DYN, NEG, true, K>(a, m, ni, y, ystride, sc, L % a.ns, L / a.ns);
-    else
-        spread_models_body<TT, DYN, NEG, false, K>(a, m, ni, y, ystride, sc, L % a.ns, L / a.ns);
-}
-
-// K models (K > 1: model k's traces at d + k*dstride, its image at
-// m + k*mstride); groups of NP tiles per wave
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP, int K>
-__global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD *__restrict__ d,
-                                                              TD *__restrict__ m,
-                                                              int64_t dstride, int64_t mstride)
-{
-    gather_tiles<TD, TT, DYN, NEG, JW, P, NP, K>(a, d, m, xcd_remap(blockIdx.x, gridDim.x),
-                                                 dstride, mstride);
-}
 
 template <typename TD>
 __global__ __launch_bounds__(kConvThreads) void conv_kernel(const TD *__restrict__ x, int64_t xstride,
@@ -298,329 +265,6 @@ int check_plan(const kx_plan *p)
     return KX_OK;
 }
 
-constexpr int kRblocks[] = {16, 8, 4, 2, 1};
-constexpr int kLdsPerCu = 160 * 1024;
-constexpr int kMaxWavesPerCu = 32;
-
-size_t spread_lds_bytes(int rb, int nt) { return (size_t)rb * nt * 8; }
-
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD, int RB>
-int launch_spread_rb(const KArgs &a, const void *m, void *y, const SpreadScale *sc,
-                     hipStream_t st)
-{
-    const size_t lds = spread_lds_bytes(RB, a.nt);
-    const int nrb = (a.nr + RB - 1) / RB;
-    const int nblk = a.ns * nrb;
-    allow_lds(reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, RB, XS>), lds);
-    hipLaunchKernelGGL((spread_kernel<TD, TT, DYN, NEG, ORD, RB, XS>), dim3(nblk),
-                       dim3(kSpreadBlock), lds, st, a, reinterpret_cast<const TD *>(m),
-                       reinterpret_cast<TD *>(y), sc);
-    KX_LAUNCHED("spread");
-    return KX_OK;
-}
-
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD>
-const void *spread_fn(int rb)
-{
-    switch (rb) {
-    case 16:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 16, XS>);
-    case 8:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 8, XS>);
-    case 4:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 4, XS>);
-    case 2:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, ORD, 2, XS>);
-    default:
-        return reinterpret_cast<const void *>(&spread_kernel<TD, TT, DYN, NEG, kOrdNatural, 1, XS>);
-    }
-}
-
-// Workgroups of one spread kernel resident per CU (registers, LDS, waves);
-// without a device, the LDS and wave limits alone.
-int spread_per_cu(const void *fn, int rb, int nt)
-{
-    const size_t lds = spread_lds_bytes(rb, nt);
-    allow_lds(fn, lds);
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kSpreadBlock, lds) != hipSuccess ||
-        n <= 0) {
-        (void)hipGetLastError();
-        n = std::min((int)(kLdsPerCu / lds), kMaxWavesPerCu / kSpreadWaves);
-    }
-    return std::max(n, 1);
-}
-
-// Receivers per workgroup.  A requested value is rounded down to a supported
-// one whose traces fit in LDS.  Automatic: the block that keeps the most
-// traces resident per CU, weighted by how far the resident waves (up to 24)
-// can hide the latency of the table loads; ties go to the larger block,
-// which re-reads the image and the source table less often.
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD>
-int pick_rblock(int requested, int nr, int nt, int max_rb)
-{
-    auto fits = [&](int rb) {
-        return rb <= max_rb && spread_lds_bytes(rb, nt) <= lds_budget();
-    };
-    if (requested > 0) {
-        for (int rb : kRblocks)
-            if (rb <= requested && fits(rb))
-                return rb;
-        return 1;
-    }
-    int best = 1;
-    double best_score = -1.0;
-    for (int rb : kRblocks) {
-        if (!fits(rb) || (rb > 1 && rb / 2 >= nr))
-            continue;
-        const int per_cu = spread_per_cu(spread_fn<TD, TT, DYN, NEG, XS, ORD>(rb), rb, nt);
-        const double waves = std::min(24.0, (double)per_cu * kSpreadWaves);
-        const double score = waves / 24.0 * per_cu * std::min(rb, nr);
-        if (score > best_score * 1.0001) {
-            best = rb;
-            best_score = score;
-        }
-    }
-    return best;
-}
-
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD>
-int launch_spread_o(KArgs a, int requested, int max_rb, const void *m, void *y,
-                    const SpreadScale *sc, hipStream_t st)
-{
-    a.rblock = pick_rblock<TD, TT, DYN, NEG, XS, ORD>(requested, a.nr, a.nt, max_rb);
-    switch (a.rblock) {
-    case 16:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 16>(a, m, y, sc, st);
-    case 8:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 8>(a, m, y, sc, st);
-    case 4:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 4>(a, m, y, sc, st);
-    case 2:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, ORD, 2>(a, m, y, sc, st);
-    default:
-        return launch_spread_rb<TD, TT, DYN, NEG, XS, kOrdNatural, 1>(a, m, y, sc, st);
-    }
-}
-
-template <typename TD, typename TT, bool DYN, bool NEG, bool XS>
-int launch_spread_t(const KArgs &a, int order, int requested, int max_rb, const void *m, void *y,
-                    const SpreadScale *sc, hipStream_t st)
-{
-    switch (order) {
-    case KX_ORDER_ROTATED:
-        return launch_spread_o<TD, TT, DYN, NEG, XS, kOrdRotated>(a, requested, max_rb, m, y, sc,
-                                                                  st);
-    case KX_ORDER_GROUPED:
-        return launch_spread_o<TD, TT, DYN, NEG, XS, kOrdGrouped>(a, requested, max_rb, m, y, sc,
-                                                                  st);
-    default:
-        return launch_spread_o<TD, TT, DYN, NEG, XS, kOrdNatural>(a, requested, max_rb, m, y, sc,
-                                                                  st);
-    }
-}
-
-
-template <typename TD, typename TT>
-int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream_t st)
-{
-    if (spread_lds_bytes(1, p->nt) > lds_budget())
-        return kx_fail(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
-    if (sizeof(TD) == 4 && !p->aux)
-        return kx_fail(KX_EINVAL, "spread: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
-    // registers per lane grow with the block: keep two workgroups per CU
-    const int max_rb = sizeof(TD) == 8 ? 4 : (p->dynamic ? 8 : 16);
-    SpreadScale *sc = reinterpret_cast<SpreadScale *>(p->aux);
-    if constexpr (sizeof(TD) == 4) {
-        // fixed-point scale of this launch from max|m|
-        const int64_t ni = (int64_t)p->nx * p->ny * p->nz;
-        float *part = reinterpret_cast<float *>(p->aux) + 16;
-        const int nb = (int)std::min<int64_t>(kScaleBlocks, (ni + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(absmax_partial_kernel, dim3(nb), dim3(kBlock), 0, st,
-                           reinterpret_cast<const float *>(m), ni, part);
-        KX_LAUNCHED("spread scale");
-        hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp, sc);
-        KX_LAUNCHED("spread scale");
-    }
-    // A workgroup blocking sources (float32 operators) runs the spread with
-    // the two sides' tables exchanged and writes its traces to the
-    // transposed positions: T = os + or and the integer bases add in either
-    // order, and the dynamic kernel (XS) forms the source-side products per
-    // step, so the traces are bitwise those of blocks of receivers.
-    // Automatic: a 2D kinematic plan blocks the side with fewer members,
-    // whose rows then serve many workgroups through L2 (13% faster on a
-    // 61 x 301 geometry, either way round); 3D and dynamic plans block
-    // receivers (measured no faster otherwise).
-    int side = p->sside;
-    if (side == KX_SIDE_AUTO)
-        side = (!p->dynamic && p->ny == 1 && p->ns < p->nr) ? KX_SIDE_SOURCES : KX_SIDE_RECEIVERS;
-    const bool xs = sizeof(TD) == 4 && side == KX_SIDE_SOURCES;
-    if (xs)
-        exchange_sides(a);
-    // automatic lane order: rotated when a step loads one receiver table,
-    // natural when it loads three (dynamic weights), where the scattered
-    // loads cost more than the collisions they avoid
-    int order = p->sorder;
-    if (order == KX_ORDER_AUTO)
-        order = p->dynamic ? KX_ORDER_NATURAL : KX_ORDER_ROTATED;
-    // exchanged-side dynamic kernels exist for float32 operators only (xs
-    // is false otherwise)
-    constexpr bool kXS = sizeof(TD) == 4;
-    if (p->dynamic && xs)
-        return p->neg_trav ? launch_spread_t<TD, TT, true, true, kXS>(a, order, p->rblock, max_rb,
-                                                                      m, y, sc, st)
-                           : launch_spread_t<TD, TT, true, false, kXS>(a, order, p->rblock,
-                                                                       max_rb, m, y, sc, st);
-    if (p->dynamic)
-        return p->neg_trav ? launch_spread_t<TD, TT, true, true, false>(a, order, p->rblock,
-                                                                        max_rb, m, y, sc, st)
-                           : launch_spread_t<TD, TT, true, false, false>(a, order, p->rblock,
-                                                                         max_rb, m, y, sc, st);
-    return p->neg_trav ? launch_spread_t<TD, TT, false, true, false>(a, order, p->rblock, max_rb,
-                                                                     m, y, sc, st)
-                       : launch_spread_t<TD, TT, false, false, false>(a, order, p->rblock,
-                                                                      max_rb, m, y, sc, st);
-}
-
-// sources per wave and pass (measured): every pass re-reads the receiver
-// table -- three tables when dynamic -- and more sources hold more
-// registers, lowering occupancy
-template <bool DYN, int NP, int K> struct GatherJW {
-    static constexpr int v = (DYN && NP == 1 && K == 1) ? 8 : 4;
-};
-
-template <typename TD, typename TT, bool DYN, bool NEG, int P, int NP, int K>
-int launch_gather_p(KArgs a, const void *d, int64_t dstride, void *m, int64_t mstride,
-                    hipStream_t st)
-{
-    constexpr int JW = GatherJW<DYN, NP, K>::v;
-    constexpr int NG = kGatherWaves / P;
-    if ((int64_t)JW * P * a.nr * a.nt >= ((int64_t)1 << 31))
-        return kx_fail(KX_ESHAPE, "gather: %d sources x nr x nt exceeds 2^31 samples", JW * P);
-    set_tile_groups(a, NP);
-    const int nblk = (a.ngroups + NG - 1) / NG;
-    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P, NP, K>), dim3(nblk),
-                       dim3(kGatherBlock), 0, st, a, reinterpret_cast<const TD *>(d),
-                       reinterpret_cast<TD *>(m), dstride, mstride);
-    KX_LAUNCHED("gather");
-    return KX_OK;
-}
-
-// Waves per tile group: the fewest that give every CU kGatherFill waves.
-constexpr int kGatherFill = 24;
-
-int device_cus()
-{
-    int cus = 256, dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
-        (void)hipGetLastError();
-        cus = 256;
-    }
-    return cus;
-}
-
-// Tiles per wave (NP): requested (rounded down to 1, 2 or 4), or the most
-// whose groups still give every CU kGatherFill waves with at most
-// kGatherWaves waves per group.
-int gather_tiles_per_wave(const KArgs &a, int gtiles)
-{
-    if (gtiles > 0)
-        return gtiles >= 4 ? 4 : (gtiles >= 2 ? 2 : 1);
-    const int64_t want = (int64_t)kGatherFill * device_cus();
-    for (int np : {4, 2}) {
-        KArgs b = a;
-        set_tile_groups(b, np);
-        if ((int64_t)b.ngroups * kGatherWaves >= want)
-            return np;
-    }
-    return 1;
-}
-
-int gather_split(const KArgs &a, int np, int gsplit)
-{
-    int P = 1;
-    if (gsplit > 0) {
-        while (P * 2 <= std::min(gsplit, kGatherWaves))
-            P *= 2;
-    } else {
-        KArgs b = a;
-        set_tile_groups(b, np);
-        const int64_t want = (int64_t)kGatherFill * device_cus();
-        while (P < kGatherWaves && (int64_t)b.ngroups * P < want)
-            P *= 2;
-    }
-    return P;
-}
-
-template <typename TD, typename TT, bool DYN, bool NEG, int NP, int K>
-int launch_gather_np(const KArgs &a, int P, const void *d, void *m, hipStream_t st,
-                     int64_t dstride, int64_t mstride)
-{
-    switch (P) {
-    case 1:
-        return launch_gather_p<TD, TT, DYN, NEG, 1, NP, K>(a, d, dstride, m, mstride, st);
-    case 2:
-        return launch_gather_p<TD, TT, DYN, NEG, 2, NP, K>(a, d, dstride, m, mstride, st);
-    default:
-        return launch_gather_p<TD, TT, DYN, NEG, 4, NP, K>(a, d, dstride, m, mstride, st);
-    }
-}
-
-template <typename TD, typename TT, bool DYN, bool NEG, int K = 1>
-int launch_gather_t(const KArgs &a, int gsplit, int gtiles, const void *d, void *m,
-                    hipStream_t st, int64_t dstride = 0, int64_t mstride = 0)
-{
-    const int np = gather_tiles_per_wave(a, gtiles);
-    const int P = gather_split(a, np, gsplit);
-    switch (np) {
-    case 4:
-        return launch_gather_np<TD, TT, DYN, NEG, 4, K>(a, P, d, m, st, dstride, mstride);
-    case 2:
-        return launch_gather_np<TD, TT, DYN, NEG, 2, K>(a, P, d, m, st, dstride, mstride);
-    default:
-        return launch_gather_np<TD, TT, DYN, NEG, 1, K>(a, P, d, m, st, dstride, mstride);
-    }
-}
-
-template <typename TD, typename TT>
-int launch_gather_d(const kx_plan *p, const KArgs &a, const void *d, void *m, hipStream_t st)
-{
-    const int g = p->gsplit, n = p->gtiles;
-    if (p->dynamic)
-        return p->neg_trav ? launch_gather_t<TD, TT, true, true>(a, g, n, d, m, st)
-                           : launch_gather_t<TD, TT, true, false>(a, g, n, d, m, st);
-    return p->neg_trav ? launch_gather_t<TD, TT, false, true>(a, g, n, d, m, st)
-                       : launch_gather_t<TD, TT, false, false>(a, g, n, d, m, st);
-}
-
-// K float32 models per launch, K in {4, 2}, then one by one.
-template <typename TT, bool DYN, bool NEG>
-int launch_gather_models(const kx_plan *p, const KArgs &a, const float *d, float *m, int nmodel,
-                         hipStream_t st)
-{
-    const int64_t ni = (int64_t)p->nx * p->ny * p->nz;
-    const int64_t dstride = (int64_t)p->ns * p->nr * p->nt;
-    const int g = p->gsplit, n = p->gtiles;
-    int j = 0, rc = KX_OK;
-    while (j < nmodel && rc == KX_OK) {
-        const int left = nmodel - j;
-        const float *dj = d + j * dstride;
-        float *mj = m + j * ni;
-        if (left >= 4) {
-            rc = launch_gather_t<float, TT, DYN, NEG, 4>(a, g, n, dj, mj, st, dstride, ni);
-            j += 4;
-        } else if (left >= 2) {
-            rc = launch_gather_t<float, TT, DYN, NEG, 2>(a, g, n, dj, mj, st, dstride, ni);
-            j += 2;
-        } else {
-            rc = launch_gather_t<float, TT, DYN, NEG, 1>(a, g, n, dj, mj, st);
-            j += 1;
-        }
-    }
-    return rc;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -692,157 +336,6 @@ extern "C" int kx_tile_table(const kx_plan *geom, const double *tabT, int32_t n,
         return kx_fail(KX_ETYPE, "tile_table: unknown format %d", fmt);
     KX_LAUNCHED("tile_table");
     return KX_OK;
-}
-
-extern "C" int kx_spread(const kx_plan *p, const void *m, void *traces, void *stream)
-{
-    int rc = check_plan(p);
-    if (rc)
-        return rc;
-    if (!m || !traces)
-        return kx_fail(KX_EINVAL, "spread: NULL buffer");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const KArgs a = make_args(p);
-    if (p->dtype == KX_F64)
-        return launch_spread_d<double, double>(p, a, m, traces, st);
-    if (p->tab_fmt == KX_F16)
-        return launch_spread_d<float, __half>(p, a, m, traces, st);
-    return launch_spread_d<float, float>(p, a, m, traces, st);
-}
-
-namespace {
-
-constexpr int kMaxModels = 64;
-constexpr size_t kModelScaleOffset = 2048;  // bytes into plan.aux
-
-template <typename TT, bool DYN, bool NEG, int K>
-void launch_models_k(const KArgs &a, const float *m, int64_t ni, float *y, int64_t ystride,
-                     const SpreadScale *sc, hipStream_t st)
-{
-    const size_t lds = spread_lds_bytes(K, a.nt);
-    hipLaunchKernelGGL((spread_models_kernel<TT, DYN, NEG, K>), dim3(a.ns * a.nr),
-                       dim3(kSpreadBlock), lds, st, a, m, ni, y, ystride, sc);
-}
-
-template <typename TT, bool DYN, bool NEG>
-int launch_models(const kx_plan *p, const KArgs &a, const float *m, float *y, int nmodel,
-                  hipStream_t st)
-{
-    const int64_t ni = (int64_t)p->nx * p->ny * p->nz;
-    const int64_t ystride = (int64_t)p->ns * p->nr * p->nt;
-    SpreadScale *scs = reinterpret_cast<SpreadScale *>(static_cast<char *>(p->aux) +
-                                                       kModelScaleOffset);
-    float *part = reinterpret_cast<float *>(p->aux) + 16;
-    const int nb = (int)std::min<int64_t>(kScaleBlocks, (ni + kBlock - 1) / kBlock);
-    for (int j = 0; j < nmodel; ++j) {
-        hipLaunchKernelGGL(absmax_partial_kernel, dim3(nb), dim3(kBlock), 0, st, m + j * ni, ni,
-                           part);
-        hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp,
-                           scs + j);
-    }
-    KX_LAUNCHED("spread models scale");
-    int j = 0;
-    while (j < nmodel) {
-        const int left = nmodel - j;
-        auto fits = [&](int k) {
-            return k <= left && spread_lds_bytes(k, a.nt) <= (size_t)kLdsBudget;
-        };
-        const float *mj = m + j * ni;
-        float *yj = y + j * ystride;
-        if (fits(8)) {
-            launch_models_k<TT, DYN, NEG, 8>(a, mj, ni, yj, ystride, scs + j, st);
-            j += 8;
-        } else if (fits(4)) {
-            launch_models_k<TT, DYN, NEG, 4>(a, mj, ni, yj, ystride, scs + j, st);
-            j += 4;
-        } else if (fits(2)) {
-            launch_models_k<TT, DYN, NEG, 2>(a, mj, ni, yj, ystride, scs + j, st);
-            j += 2;
-        } else {
-            const int rc = launch_spread_d<float, TT>(p, a, mj, yj, st);
-            if (rc)
-                return rc;
-            j += 1;
-        }
-        KX_LAUNCHED("spread models");
-    }
-    return KX_OK;
-}
-
-}  // namespace
-
-extern "C" int kx_spread_models(const kx_plan *p, const void *m, void *traces, int32_t nmodel,
-                                void *stream)
-{
-    int rc = check_plan(p);
-    if (rc)
-        return rc;
-    if (!m || !traces)
-        return kx_fail(KX_EINVAL, "spread_models: NULL buffer");
-    if (p->dtype != KX_F32)
-        return kx_fail(KX_ETYPE, "spread_models: float32 operators only");
-    if (nmodel < 1 || nmodel > kMaxModels)
-        return kx_fail(KX_ESHAPE, "spread_models: nmodel=%d outside [1, %d]", nmodel, kMaxModels);
-    if (!p->aux)
-        return kx_fail(KX_EINVAL, "spread_models: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
-    if (spread_lds_bytes(1, p->nt) > (size_t)kLdsBudget)
-        return kx_fail(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const KArgs a = make_args(p);
-    const float *mf = reinterpret_cast<const float *>(m);
-    float *yf = reinterpret_cast<float *>(traces);
-    if (p->tab_fmt == KX_F16)
-        return p->dynamic ? (p->neg_trav ? launch_models<__half, true, true>(p, a, mf, yf, nmodel, st)
-                                         : launch_models<__half, true, false>(p, a, mf, yf, nmodel, st))
-                          : (p->neg_trav ? launch_models<__half, false, true>(p, a, mf, yf, nmodel, st)
-                                         : launch_models<__half, false, false>(p, a, mf, yf, nmodel, st));
-    return p->dynamic ? (p->neg_trav ? launch_models<float, true, true>(p, a, mf, yf, nmodel, st)
-                                     : launch_models<float, true, false>(p, a, mf, yf, nmodel, st))
-                      : (p->neg_trav ? launch_models<float, false, true>(p, a, mf, yf, nmodel, st)
-                                     : launch_models<float, false, false>(p, a, mf, yf, nmodel, st));
-}
-
-extern "C" int kx_gather(const kx_plan *p, const void *traces, void *m, void *stream)
-{
-    int rc = check_plan(p);
-    if (rc)
-        return rc;
-    if (!m || !traces)
-        return kx_fail(KX_EINVAL, "gather: NULL buffer");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const KArgs a = make_args(p);
-    if (p->dtype == KX_F64)
-        return launch_gather_d<double, double>(p, a, traces, m, st);
-    if (p->tab_fmt == KX_F16)
-        return launch_gather_d<float, __half>(p, a, traces, m, st);
-    return launch_gather_d<float, float>(p, a, traces, m, st);
-}
-
-extern "C" int kx_gather_models(const kx_plan *p, const void *traces, void *m, int32_t nmodel,
-                                void *stream)
-{
-    int rc = check_plan(p);
-    if (rc)
-        return rc;
-    if (!m || !traces)
-        return kx_fail(KX_EINVAL, "gather_models: NULL buffer");
-    if (p->dtype != KX_F32)
-        return kx_fail(KX_ETYPE, "gather_models: float32 operators only");
-    if (nmodel < 1)
-        return kx_fail(KX_ESHAPE, "gather_models: nmodel=%d", nmodel);
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const KArgs a = make_args(p);
-    const float *df = reinterpret_cast<const float *>(traces);
-    float *mf = reinterpret_cast<float *>(m);
-    if (p->tab_fmt == KX_F16)
-        return p->dynamic ? (p->neg_trav ? launch_gather_models<__half, true, true>(p, a, df, mf, nmodel, st)
-                                         : launch_gather_models<__half, true, false>(p, a, df, mf, nmodel, st))
-                          : (p->neg_trav ? launch_gather_models<__half, false, true>(p, a, df, mf, nmodel, st)
-                                         : launch_gather_models<__half, false, false>(p, a, df, mf, nmodel, st));
-    return p->dynamic ? (p->neg_trav ? launch_gather_models<float, true, true>(p, a, df, mf, nmodel, st)
-                                     : launch_gather_models<float, true, false>(p, a, df, mf, nmodel, st))
-                      : (p->neg_trav ? launch_gather_models<float, false, true>(p, a, df, mf, nmodel, st)
-                                     : launch_gather_models<float, false, false>(p, a, df, mf, nmodel, st));
 }
 
 extern "C" int kx_conv_rows(const void *x, int64_t xstride, void *y, int64_t ntr, int32_t nt,
@@ -921,6 +414,10 @@ extern "C" int kx_forward(const kx_plan *p, const void *m, void *d, void *work, 
         return kx_fail(KX_EINVAL, "forward: wavelet missing");
     if (!work)
         return kx_fail(KX_EINVAL, "forward: workspace missing");
+    // one launch where the workgroups can convolve their traces in LDS
+    rc = kx_spread_wavelet(p, m, d, stream);
+    if (rc != KX_ENOFUSE)
+        return rc;
     rc = kx_spread(p, m, work, stream);
     if (rc)
         return rc;

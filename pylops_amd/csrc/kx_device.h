@@ -60,6 +60,12 @@ template <typename T, int N> struct alignas(sizeof(T) * N) VecN {
     T v[N];
 };
 
+// Element type of the source-side offsets: float16 plans keep the source
+// tables (a few rows per tile) in float32, so that a traveltime sum carries
+// one float16 rounding, the receiver's.
+template <typename TT> struct SrcTab { using type = TT; };
+template <> struct SrcTab<__half> { using type = float; };
+
 __device__ __forceinline__ float ld_off(const float *p) { return *p; }
 __device__ __forceinline__ float ld_off(const __half *p) { return __half2float(*p); }
 __device__ __forceinline__ double ld_off(const double *p) { return *p; }
@@ -247,45 +253,39 @@ __device__ __forceinline__ int vgpr_zero()
 // Lane roles.  Lane p holds point p of the tile and reads the source side
 // of its point once per tile.  At step r (r < RB) it treats receiver j of
 // the workgroup's block:
-//   natural  j = r: the table loads of a step are coalesced rows;
-//   rotated  j = (p + r) mod RB: one atomic instruction covers all RB trace
-//            rows at 64 / RB points RB apart each;
-//   grouped  j = (p / 16 + r) mod RB: each group of 16 lanes (a run of 16
-//            points along z in the default tiles) treats one receiver, so a
-//            load reads four 64-byte row segments, as many as a coalesced
-//            row, and one atomic instruction covers four trace rows.
-// Samples of neighbouring points of one trace often coincide, and the lanes
-// of an LDS atomic that hit one address are serialised: the rotated order
-// trades scattered table loads for fewer such collisions, the grouped order
-// keeps the loads of the natural one and splits the collisions four ways.
+//   natural   j = r: the table loads of a step are coalesced rows;
+//   rotated   j = (p + r) mod RB: one atomic instruction covers all RB trace
+//             rows at 64 / RB points RB apart each; a step's table load
+//             reads a scattered element of every row.
+// Samples of neighbouring points of one trace often coincide, and an LDS
+// atomic instruction whose lanes crowd a few banks of one row is
+// serialised: the rotated order spreads each instruction over RB rows.
 constexpr int kOrdNatural = 0;
 constexpr int kOrdRotated = 1;
-constexpr int kOrdGrouped = 2;
+
+// The receiver lane p treats at step r.
 template <int RB, int ORD> __device__ __forceinline__ int step_receiver(int lane, int r)
 {
-    if constexpr (ORD == kOrdRotated)
-        return (lane + r) & (RB - 1);
-    else if constexpr (ORD == kOrdGrouped)
-        return ((lane >> 4) + r) & (RB - 1);
-    else
+    if constexpr (ORD == kOrdNatural)
         return r;
+    else
+        return (lane + r) & (RB - 1);
 }
 
 // Receiver bases held per lane: the base of receiver (lane mod RB) of the
 // block, one load for all RB receivers, handed to the lanes that need it by
-// a lane read (natural order) or a quad permutation (rotated order, RB <= 4);
-// rotated blocks of 8 and 16 receivers and grouped blocks load one base per
+// a lane read (natural order) or a quad permutation (rotated order,
+// RB <= 4); rotated blocks of 8 and 16 receivers load one base per
 // step instead.
 template <int ORD, int RB> struct BaseLoads {
-    static constexpr int n =
-        ((ORD == kOrdRotated && RB >= 8) || (ORD == kOrdGrouped && RB >= 2)) ? RB : 1;
+    static constexpr int n = (ORD == kOrdRotated && RB >= 8) ? RB : 1;
 };
 
 // One lane's share of one tile: the image value and the table entries of
 // its point, receivers in the lane's step order.
 template <typename TD, typename TT, int ORD, int RB> struct SpreadTile {
     TD mv;
-    TT os;       // raw table elements: converted at use, so that a prefetch
+    typename SrcTab<TT>::type os;  // raw table elements: converted at use, so that a prefetch
     TT orr[RB];  // never waits for them
     TD ps, qs;
     TD pr[RB], qr[RB];
@@ -299,7 +299,7 @@ __device__ __forceinline__ int step_base(const int (&br)[BaseLoads<ORD, RB>::n],
 {
     if constexpr (BaseLoads<ORD, RB>::n > 1) {
         return br[r];
-    } else if constexpr (ORD != kOrdRotated || RB == 1) {
+    } else if constexpr (ORD == kOrdNatural || RB == 1) {
         // every lane treats receiver r: the base held by lane r
         return __builtin_amdgcn_readlane(br[0], r);
     } else if constexpr (RB == 2) {
@@ -336,7 +336,7 @@ __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c,
     L.mv = valid ? (m + orig)[(unsigned)rel] : TD(0);
     const size_t ts = (size_t)t * a.ns + s;        // (tile, source) row
     const size_t tr = (size_t)t * a.nr + r0;       // (tile, first receiver) row
-    L.os = (reinterpret_cast<const TT *>(a.off_s) + ts * kWave)[(unsigned)lane];
+    L.os = (reinterpret_cast<const typename SrcTab<TT>::type *>(a.off_s) + ts * kWave)[(unsigned)lane];
     L.bs = a.base_s[ts + vz];
     if constexpr (DYN) {
         L.ps = (reinterpret_cast<const TD *>(a.wp_s) + ts * kWave)[(unsigned)lane];
@@ -380,6 +380,19 @@ __device__ __forceinline__ int floor_int(float x)
     return i;
 }
 
+constexpr int kConvThreads = 128;
+constexpr int kConvPer = 4;
+constexpr int kConvOut = kConvThreads * kConvPer;
+// filter length padded to a whole number of 4-element groups
+__host__ __device__ __forceinline__ int conv_taps_padded(int nh) { return (nh + 3) & ~3; }
+
+// LDS accumulation of the forward's traces (ACC): kAccFix, 64-bit fixed
+// point with one 64-bit integer atomic per sample (float32 data), or
+// kAccF64, float64 atomics (float64 data; float32 launches whose model
+// holds inf or NaN).
+constexpr int kAccF64 = 0;
+constexpr int kAccFix = 1;
+
 // One contribution of weight w at time T samples past the integer base:
 // w (1 - f) to sample k and w f to sample k + 1 of the LDS trace at lrow,
 // k = floor(T) + base, f = T - floor(T).  Lanes with lrow < 0 (no receiver),
@@ -388,10 +401,9 @@ __device__ __forceinline__ int floor_int(float x)
 // and one fract instruction: fract(T) is T - floor(T) rounded as the
 // subtraction rounds it, except that a result that would round to 1 (T in
 // (-2^-24, 0)) stays just below 1.
-template <typename TD, typename TC, bool NEG, bool FIX>
+template <typename TD, typename TC, bool NEG, int ACC>
 __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int lrow,
-                                              unsigned ntm1, double *accd,
-                                              unsigned long long *acci)
+                                              unsigned ntm1, void *lds)
 {
     TC f;
     int k;
@@ -414,13 +426,15 @@ __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int
     if (on && lrow >= 0 && (unsigned)k < ntm1 && w != TD(0)) {
         const int it = lrow + k;
         const TD w1 = w * (TD)f;
-        if constexpr (FIX) {
+        if constexpr (ACC == kAccFix) {
             // w (1 - f) in one rounding; fixed point by floor(x + 1/2) in
             // one conversion
+            unsigned long long *acci = static_cast<unsigned long long *>(lds);
             const TD w0 = fma(-w, (TD)f, w);
             atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
             atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
         } else {
+            double *accd = static_cast<double *>(lds);
             const TD w0 = w * (TD)(TC(1) - f);
             atomicAdd(&accd[it], (double)w0);
             atomicAdd(&accd[it + 1], (double)w1);
@@ -433,13 +447,12 @@ __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int
 // products formed once per tile; with the sides exchanged (XS: the block
 // members are sources) the same products are formed per step, so that the
 // weights round as they do with blocks of receivers.
-template <typename TD, typename TT, bool DYN, bool NEG, int ORD, bool FIX, int RB,
+template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int ACC, int RB,
           bool XS = false>
 __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, RB> &cur, int iz,
                                                   int e,
                                                   const int (&lrow)[RB], const int (&izm)[RB],
-                                                  unsigned ntm1, double *accd,
-                                                  unsigned long long *acci)
+                                                  unsigned ntm1, void *lds)
 {
     using TC = typename TimeType<TD>::type;
     if (__ballot(cur.mv != TD(0)) == 0)
@@ -447,7 +460,7 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, 
     const TC os = (TC)ld_off(&cur.os);
     // the image value, scaled by 2^e for the fixed-point sum
     TD mv = cur.mv;
-    if constexpr (FIX)
+    if constexpr (ACC == kAccFix)
         mv = ldexpf(mv, e);
     TD mps = mv, mqs = TD(0);
     if constexpr (DYN && !XS) {
@@ -460,42 +473,42 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, 
         TD w = mv;
         bool on = true;
         if constexpr (DYN) {
-            if constexpr (XS)  // (ps, qs) here are the receiver's, pr[r], qr[r] the source's
+            if constexpr (XS)  // (ps, qs) here are the receiver's, pr, qr the source's
                 w = fma(mv * cur.qr[r], cur.qs, (mv * cur.pr[r]) * cur.ps);
             else
                 w = fma(mqs, cur.qr[r], mps * cur.pr[r]);
             on = iz >= izm[r];
         }
-        spread_sample<TD, TC, NEG, FIX>(T, cur.bs + step_base<ORD, RB>(cur.br, r), w, on,
-                                        lrow[r], ntm1, accd, acci);
+        spread_sample<TD, TC, NEG, ACC>(T, cur.bs + step_base<ORD, RB>(cur.br, r), w, on,
+                                        lrow[r], ntm1, lds);
     }
 }
 
 // Zeroes the ntr samples of a workgroup's LDS traces (0.0 and integer 0
 // share the bit pattern).
-__device__ __forceinline__ void traces_clear(unsigned long long *acci, int ntr)
+__device__ __forceinline__ void traces_clear(void *lds, int ntr)
 {
+    unsigned long long *p = static_cast<unsigned long long *>(lds);
     for (int i = threadIdx.x; i < ntr; i += blockDim.x)
-        acci[i] = 0ull;
+        p[i] = 0ull;
     __syncthreads();
 }
 
 // Writes the workgroup's LDS traces (ntr = rows * nt samples) to y: row j
 // to trace first + j * trb; fixed point scaled back by 2^-e, or float64
 // sums rounded to TD.
-template <typename TD, bool FIX>
+template <typename TD, int ACC>
 __device__ __forceinline__ void traces_store(TD *__restrict__ y, size_t first, int trb, int nt,
-                                             int ntr, int e, const double *accd,
-                                             const unsigned long long *acci)
+                                             int ntr, int e, const void *lds)
 {
     __syncthreads();
     TD *yo = y + first * nt;
     for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
         TD v;
-        if constexpr (FIX)
-            v = (TD)ldexp((double)(long long)acci[i], -e);
+        if constexpr (ACC == kAccFix)
+            v = (TD)ldexp((double)(long long)static_cast<const unsigned long long *>(lds)[i], -e);
         else
-            v = (TD)accd[i];
+            v = (TD) static_cast<const double *>(lds)[i];
         if (trb == 1) {
             yo[i] = v;  // the block's traces are adjacent
         } else {
@@ -506,20 +519,133 @@ __device__ __forceinline__ void traces_store(TD *__restrict__ y, size_t first, i
     __syncthreads();  // the traces are read out before a next item clears them
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int ORD, bool FIX, int RB,
+// The forward's wavelet convolution on the workgroup's LDS traces (the
+// reference convolves after the spread, K:1048-1094 / C1D:99-115): the
+// 64-bit sums become float32 rows padded with zeros in the same LDS --
+// converted in passes of kFusePass samples per thread, each pass read into
+// registers before any of it is written, so that a pass writes only over
+// samples already read (fused_conv_ok checks this for the launch's shape)
+// -- and each row is convolved as kx_conv convolves a trace, with the
+// operations in the same order, so the data are bitwise those of the
+// spread followed by kx_conv.  Row j holds x[p - L] at position p (L =
+// (nh - 1) / 2, zeros outside the trace) in S = fused_row(nt, nh) floats;
+// the flipped filter, zero padded to whole 4-tap groups, follows the rows.
+constexpr int kFusePass = 8;
+
+__host__ __device__ __forceinline__ int fused_row(int nt, int nh)
+{
+    return (nt + conv_taps_padded(nh) + 3 + 3) & ~3;
+}
+
+// Whether a launch of rb-trace workgroups with lds bytes of LDS can convolve
+// in place: the float rows and the filter fit, and no conversion pass
+// writes past the first sample of the next pass (the samples are 8-byte
+// sums, the rows 4-byte floats).
+inline bool fused_conv_ok(int rb, int nt, int nh, size_t lds, int block)
+{
+    const int64_t S = fused_row(nt, nh), L = (nh - 1) / 2;
+    if (((int64_t)rb * S + conv_taps_padded(nh)) * 4 > (int64_t)lds)
+        return false;
+    const int64_t ntr = (int64_t)rb * nt, B = (int64_t)kFusePass * block;
+    for (int64_t q = 0; (q + 1) * B < ntr; ++q) {
+        const int64_t i = (q + 1) * B - 1;  // last sample of pass q
+        const int64_t j = i / nt, n = i - j * nt;
+        if ((j * S + L + n + 1) * 4 > (q + 1) * B * 8)
+            return false;
+    }
+    return true;
+}
+
+template <typename TD, int ACC>
+__device__ __forceinline__ void traces_conv_store(TD *__restrict__ y, size_t first, int trb,
+                                                  int nt, int nrl, int e, void *lds,
+                                                  const TD *__restrict__ h, int nh)
+{
+    if constexpr (sizeof(TD) == 4) {
+        const int ntr = nrl * nt;
+        const int S = fused_row(nt, nh), L = (nh - 1) / 2, nhp = conv_taps_padded(nh);
+        float *win = static_cast<float *>(lds);
+        const unsigned long long *acci = static_cast<const unsigned long long *>(lds);
+        const double *accd = static_cast<const double *>(lds);
+        __syncthreads();  // every contribution added
+        for (int p0 = 0; p0 < ntr; p0 += kFusePass * (int)blockDim.x) {
+            float v[kFusePass];
+#pragma unroll
+            for (int q = 0; q < kFusePass; ++q) {
+                const int i = p0 + q * (int)blockDim.x + (int)threadIdx.x;
+                if (i < ntr) {
+                    if constexpr (ACC == kAccF64)
+                        v[q] = (float)accd[i];
+                    else
+                        v[q] = (float)ldexp((double)(long long)acci[i], -e);
+                }
+            }
+            __syncthreads();  // the pass is read before it is overwritten
+#pragma unroll
+            for (int q = 0; q < kFusePass; ++q) {
+                const int i = p0 + q * (int)blockDim.x + (int)threadIdx.x;
+                if (i < ntr) {
+                    const int j = i / nt;
+                    win[j * S + L + (i - j * nt)] = v[q];
+                }
+            }
+            __syncthreads();
+        }
+        // zero margins of the rows, then the flipped filter
+        const int F = nrl * S;
+        for (int p = threadIdx.x; p < F; p += blockDim.x) {
+            const int n = p % S - L;
+            if (n < 0 || n >= nt)
+                win[p] = 0.f;
+        }
+        for (int k = threadIdx.x; k < nhp; k += blockDim.x)
+            win[F + k] = k < nh ? h[nh - 1 - k] : 0.f;
+        __syncthreads();
+        const int G = (nt + 3) / 4;  // groups of 4 outputs per row
+        const float4 *h4 = reinterpret_cast<const float4 *>(win + F);
+        for (int g = threadIdx.x; g < nrl * G; g += blockDim.x) {
+            const int j = g / G, n0 = (g - j * G) * 4;
+            const float4 *w4 = reinterpret_cast<const float4 *>(win + j * S + n0);
+            float acc[kConvPer] = {};
+            float4 cur = w4[0];
+            for (int gq = 0; gq < nhp / 4; ++gq) {
+                const float4 nxt = w4[gq + 1];
+                const float4 hq = h4[gq];
+                const float w[8] = {cur.x, cur.y, cur.z, cur.w, nxt.x, nxt.y, nxt.z, nxt.w};
+                const float hh[4] = {hq.x, hq.y, hq.z, hq.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int r = 0; r < kConvPer; ++r)
+                        acc[r] += hh[q] * w[q + r];
+                cur = nxt;
+            }
+            TD *yo = y + (first + (size_t)j * trb) * nt;
+#pragma unroll
+            for (int r = 0; r < kConvPer; ++r)
+                if (n0 + r < nt)
+                    yo[n0 + r] = acc[r];
+        }
+        __syncthreads();  // the rows are read before a next item clears them
+    } else {
+        traces_store<TD, ACC>(y, first, trb, nt, nrl * nt, e, lds);
+    }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int ACC, int RB,
           bool XS = false>
 __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
-                                            TD *__restrict__ y, int e, int s, int r0)
+                                            TD *__restrict__ y, int e, int s, int r0,
+                                            const TD *__restrict__ h = nullptr, int nh = 0)
 {
     extern __shared__ __align__(16) unsigned char kx_smem[];
-    double *accd = reinterpret_cast<double *>(kx_smem);
-    unsigned long long *acci = reinterpret_cast<unsigned long long *>(kx_smem);
+    void *lds = kx_smem;
 
     // one workgroup = one source s and receivers [r0, r0 + RB)
     const int nrl = min(RB, a.nr - r0);
     const int ntr = nrl * a.nt;
 
-    traces_clear(acci, ntr);
+    traces_clear(lds, ntr);
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -551,28 +677,37 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
-        spread_accumulate<TD, TT, DYN, NEG, ORD, FIX, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
-                                                              izm, ntm1, accd, acci);
+        spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
+                                                              izm, ntm1, lds);
         cur = nxt;
         c = cn;
     }
-    traces_store<TD, FIX>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, ntr, e, accd,
-                          acci);
+    // with a wavelet: convolved in LDS, y receives the data (see
+    // traces_conv_store); without: y receives the traces
+    if (h != nullptr)
+        traces_conv_store<TD, ACC>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, nrl,
+                                   e, lds, h, nh);
+    else
+        traces_store<TD, ACC>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, ntr, e,
+                              lds);
 }
 
-// One (source, receiver block) item with the launch's accumulation scale.
+// One (source, receiver block) item with the launch's accumulation scale:
+// float32 data in fixed point, or in float64 when the launch's scale says
+// so (a non-finite model); float64 data always in float64.
 template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int RB, bool XS = false>
 __device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, const SpreadScale &sc, int s,
-                                            int r0)
+                                            int r0, const TD *__restrict__ h = nullptr,
+                                            int nh = 0)
 {
     if constexpr (sizeof(TD) == 4) {
         if (sc.fixed)
-            spread_body<TD, TT, DYN, NEG, ORD, true, RB, XS>(a, m, y, sc.e, s, r0);
+            spread_body<TD, TT, DYN, NEG, ORD, kAccFix, RB, XS>(a, m, y, sc.e, s, r0, h, nh);
         else
-            spread_body<TD, TT, DYN, NEG, ORD, false, RB, XS>(a, m, y, 0, s, r0);
+            spread_body<TD, TT, DYN, NEG, ORD, kAccF64, RB, XS>(a, m, y, 0, s, r0, h, nh);
     } else {
-        spread_body<TD, TT, DYN, NEG, ORD, false, RB, XS>(a, m, y, 0, s, r0);
+        spread_body<TD, TT, DYN, NEG, ORD, kAccF64, RB, XS>(a, m, y, 0, s, r0);
     }
 }
 
@@ -587,7 +722,8 @@ __device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict
 // those of kx_spread (the sums are exact).
 template <typename TT, int K> struct ModelsTile {
     float mv[K];
-    TT os, orr;
+    typename SrcTab<TT>::type os;
+    TT orr;
     float ps, qs, pr, qr;
     int bs, br;
 };
@@ -607,7 +743,7 @@ __device__ __forceinline__ void models_load(const KArgs &a, const TileCursor &c,
         L.mv[j] = valid ? (m + j * ni + orig)[(unsigned)rel] : 0.f;
     const size_t ts = (size_t)t * a.ns + s;
     const size_t tr = (size_t)t * a.nr + r;
-    L.os = (reinterpret_cast<const TT *>(a.off_s) + ts * kWave)[(unsigned)lane];
+    L.os = (reinterpret_cast<const typename SrcTab<TT>::type *>(a.off_s) + ts * kWave)[(unsigned)lane];
     L.orr = (reinterpret_cast<const TT *>(a.off_r) + tr * kWave)[(unsigned)lane];
     L.bs = a.base_s[ts + vz];
     L.br = a.base_r[tr + vz];
@@ -771,6 +907,9 @@ __device__ __forceinline__ float absmax_wave(const float *__restrict__ part, int
 
 // A contribution is at most |m| * wamp * 2 (the truncation path of a
 // negative time weighs a sample by up to 2): e puts that bound below 2^30.
+// The scaled image values m 2^e are float32: when max|m| 2^e would not be a
+// finite float, or values 2^-40 below the maximum would no longer be normal
+// floats (weights in extreme units), the launch accumulates in float64.
 __device__ __forceinline__ SpreadScale scale_from_max(float mx, double wamp)
 {
     const double bound = 2.0 * (double)mx * wamp;
@@ -785,7 +924,8 @@ __device__ __forceinline__ SpreadScale scale_from_max(float mx, double wamp)
         int ex;
         frexp(bound, &ex);  // bound < 2^ex
         sc.e = 30 - ex;
-        sc.fixed = 1;
+        const double top = ldexp((double)mx, sc.e);
+        sc.fixed = (top <= 0x1p126 && ldexp(top, -40) >= 0x1p-126) ? 1 : 0;
     }
     return sc;
 }
@@ -911,7 +1051,8 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
         t[j] = tzi + a.ntz * (tok[j] ? txi + a.ntx * tyi : gx0 + a.ntx * gy0);
         pt[j] = tile_point(a, t[j], lane);
     }
-    const TT *offs = reinterpret_cast<const TT *>(a.off_s);
+    const typename SrcTab<TT>::type *offs =
+        reinterpret_cast<const typename SrcTab<TT>::type *>(a.off_s);
     const TT *offr = reinterpret_cast<const TT *>(a.off_r);
     const TD *wps = reinterpret_cast<const TD *>(a.wp_s);
     const TD *wqs = reinterpret_cast<const TD *>(a.wq_s);
@@ -1186,12 +1327,7 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
 // (consecutive lanes read consecutive 16 bytes: no bank conflicts) and keeps
 // the 8 window samples of two groups in registers, so 16 multiply-adds cost
 // two LDS reads.
-constexpr int kConvThreads = 128;
-constexpr int kConvPer = 4;
-constexpr int kConvOut = kConvThreads * kConvPer;
 
-// filter length padded to a whole number of 4-element groups
-__host__ __device__ __forceinline__ int conv_taps_padded(int nh) { return (nh + 3) & ~3; }
 
 // LDS elements of one convolution workgroup: filter + window + one group of
 // read-ahead

@@ -37,6 +37,7 @@ RBLOCKS = (16, 8, 4, 2, 1)
 SORDERS = (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED)
 SSIDES = (_lib.KX_SIDE_RECEIVERS, _lib.KX_SIDE_SOURCES)
 GSPLITS = (1, 2, 4)
+GTILES = (1, 2, 4)
 
 
 def _cache_path(path):
@@ -101,29 +102,30 @@ def _timer(fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def _candidates_fwd(dtype, dynamic, nt, sides=(_lib.KX_SIDE_RECEIVERS,)):
+def _candidates_fwd(dtype, dynamic, nt, sides=(_lib.KX_SIDE_RECEIVERS,), orders=SORDERS,
+                    lds=65536):
     max_rb = 4 if dtype == "float64" else (8 if dynamic else 16)
     if dtype == "float64":
         sides = (_lib.KX_SIDE_RECEIVERS,)  # float64 operators block receivers
     for rb in RBLOCKS:
-        if rb > max_rb or (rb > 1 and rb * nt * 8 > 65536):
+        if rb > max_rb or (rb > 1 and rb * nt * 8 > lds):
             continue  # more registers than two workgroups per CU, or LDS
-        for so in SORDERS if rb > 1 else (_lib.KX_ORDER_NATURAL,):
+        for so in orders if rb > 1 else (_lib.KX_ORDER_NATURAL,):
             for side in sides:
                 yield rb, so, side
 
 
 def autotune(reps=3, tiles=None, cache=None, **kw):
     """Fastest launch shapes for ``Kirchhoff(**kw)``: returns a dict with
-    ``tile``, ``rblock``, ``sorder``, ``sside``, ``gsplit`` (pass it on to
-    ``Kirchhoff``) and the measured ``ms`` (forward kernel + adjoint
-    kernel).  ``tiles`` overrides the candidate tiles; ``cache`` is the
+    ``tile``, ``rblock``, ``sorder``, ``sside``, ``gsplit``, ``gtiles``
+    (pass it on to ``Kirchhoff``) and the measured ``ms`` (forward kernel +
+    adjoint kernel).  ``tiles`` overrides the candidate tiles; ``cache`` is the
     JSON cache file (default ``$PYLOPS_AMD_TUNE_CACHE`` or
     ``~/.cache/pylops_amd/tune.json``; ``False`` disables it)."""
     from pylops_amd.waveeqprocessing.kirchhoff import Kirchhoff
     _lib.require_device()
     kw = {k: v for k, v in kw.items()
-          if k not in ("tile", "rblock", "sorder", "gsplit", "sside")}
+          if k not in ("tile", "rblock", "sorder", "gsplit", "sside", "gtiles")}
     device = kw.get("device") or torch.device("cuda", torch.cuda.current_device())
     path = None if cache is False else _cache_path(cache)
     key = _key("op", [kw], device)
@@ -132,6 +134,7 @@ def autotune(reps=3, tiles=None, cache=None, **kw):
         if hit is not None:
             hit["tile"] = tuple(hit["tile"])
             hit.setdefault("sside", _lib.KX_SIDE_AUTO)  # entries from before sside was tuned
+            hit.setdefault("gtiles", 0)  # entries from before gtiles was tuned
             return hit
     ndims = _geometry(kw)[0]
     tiles = tiles or (TILES_2D if ndims == 2 else TILES_3D)
@@ -160,18 +163,21 @@ def autotune(reps=3, tiles=None, cache=None, **kw):
                        "kx_gather")
 
         fwd = []
+        lds = torch.cuda.get_device_properties(Op.device).shared_memory_per_block
         for rb, so, side in _candidates_fwd(dtype, kw.get("dynamic", False), len(kw["t"]),
-                                            SSIDES):
+                                            SSIDES, lds=lds):
             plan.rblock, plan.sorder, plan.sside = rb, so, side
             fwd.append((_timer(spread, reps), rb, so, side))
         adj = []
         for gs in GSPLITS:
-            plan.gsplit = gs
-            adj.append((_timer(gather, reps), gs))
+            for gt in GTILES:
+                plan.gsplit, plan.gtiles = gs, gt
+                adj.append((_timer(gather, reps), gs, gt))
         tf, rb, so, side = min(fwd)
-        ta, gs = min(adj)
+        ta, gs, gt = min(adj)
         cand = {"tile": tuple(tile), "rblock": rb, "sorder": so, "sside": side, "gsplit": gs,
-                "ms": round(tf + ta, 4), "ms_forward": round(tf, 4), "ms_adjoint": round(ta, 4)}
+                "gtiles": gt, "ms": round(tf + ta, 4), "ms_forward": round(tf, 4),
+                "ms_adjoint": round(ta, 4)}
         if best is None or cand["ms"] < best["ms"]:
             best = cand
         del Op, m, traces, img
@@ -215,7 +221,9 @@ def autotune_batch(kws, reps=3, tiles=None, cache=None):
         a = torch.empty(B0.shape[1], dtype=tdt, device=B0.device)
         B0._into(m, d, False)
         fwd = []
-        for rb, so, _ in _candidates_fwd(dtype, dynamic, max(len(kw["t"]) for kw in kws)):
+        for rb, so, _ in _candidates_fwd(dtype, dynamic, max(len(kw["t"]) for kw in kws),
+                                         orders=(_lib.KX_ORDER_NATURAL,
+                                                 _lib.KX_ORDER_ROTATED)):
             B = KirchhoffBatch(ops, rblock=rb, sorder=so)
             if B.rblock != rb:
                 continue  # does not fit the batch's longest trace

@@ -37,8 +37,12 @@ _LAUNCH_ATTRS = ("rblock", "sorder", "gsplit", "sside", "gtiles")
 
 class _Table:
     """A table attribute: stored as given (numpy or torch), read back as a
-    numpy array like the reference's attributes, and any reassignment marks
-    the device copy stale (tests reassign e.g. ``Op.trav_srcs = ...``)."""
+    read-only numpy array (the reference's attributes are numpy arrays), and
+    any reassignment marks the device copy stale (tests reassign e.g.
+    ``Op.trav_srcs = ...``).  The device tables are built from the stored
+    table, so an edit in place could not reach them: it raises instead
+    (``ValueError: assignment destination is read-only``); assign a new
+    array to change a table."""
 
     def __set_name__(self, owner, name):
         self.name = name
@@ -50,7 +54,8 @@ class _Table:
             raise AttributeError(self.name)
         host = obj._tab_host.get(self.name)
         if host is None:
-            host = _host(obj._tab[self.name])
+            host = _host(obj._tab[self.name]).view()
+            host.flags.writeable = False
             obj._tab_host[self.name] = host
         return host
 
@@ -71,14 +76,15 @@ class Kirchhoff(LinearOperator):
     reference names ("numpy", "numba", "cuda"); every engine runs the HIP
     kernels.  Extra keyword-only parameters:
 
-    tabledtype : ``"float16"`` stores traveltime offsets in half precision
-        (float32 operators only); default is the operator dtype.
+    tabledtype : ``"float16"`` stores the receiver traveltime offsets in
+        half precision (float32 operators only; the source tables stay
+        float32); default is the operator dtype.
     tile : ``(tz, tx, ty)`` image tile of 64 points.
     device : torch device of the tables (default: current HIP device).
     rblock : receivers (or sources, see ``sside``) per forward workgroup
         (0 = automatic).
-    sorder : forward lane order, ``KX_ORDER_NATURAL`` (1),
-        ``KX_ORDER_ROTATED`` (2) or ``KX_ORDER_GROUPED`` (3); 0 = automatic.
+    sorder : forward lane order, ``KX_ORDER_NATURAL`` (1) or
+        ``KX_ORDER_ROTATED`` (2); 0 = automatic.
     gsplit : adjoint waves per tile group, 1, 2 or 4; 0 = automatic.
     gtiles : neighbouring image tiles per adjoint wave, sharing its staged
         trace windows, 1, 2 or 4; 0 = automatic.
@@ -341,15 +347,21 @@ class Kirchhoff(LinearOperator):
             tr = T.as_f64(self._tab["trav_recs"], dev)
             ns, nr = self.ns, self.nr
         else:
-            ts = T.as_f64(self._tab["trav"], dev)
-            tr = torch.zeros((self.ni, 1), dtype=torch.float64, device=dev)
-            ns, nr = self.nsnr, 1
+            # one combined table: the kernels' receiver side (trace
+            # isrc * nr + irec = the table's column), a zero source table
+            ts = torch.zeros((self.ni, 1), dtype=torch.float64, device=dev)
+            tr = T.as_f64(self._tab["trav"], dev)
+            ns, nr = 1, self.nsnr
         if ts.shape != (self.ni, ns) or tr.shape != (self.ni, nr):
             raise ValueError(f"traveltime tables of shape {tuple(ts.shape)} / "
                              f"{tuple(tr.shape)} do not match (ni, n) = ({self.ni}, {ns}) / "
                              f"({self.ni}, {nr})")
         neg = bool((ts.min() + tr.min()) < 0)
-        keep["off_s"], keep["base_s"] = T.encode(plan, ts, inv_dt, fmt)
+        # float16 offsets on the receiver side only: the source tables (a
+        # few rows per tile) stay float32, so that a traveltime sum carries
+        # one float16 rounding
+        keep["off_s"], keep["base_s"] = T.encode(plan, ts, inv_dt,
+                                                 _lib.KX_F32 if fmt == _lib.KX_F16 else fmt)
         keep["off_r"], keep["base_r"] = T.encode(plan, tr, inv_dt, fmt)
         del ts, tr
         wfmt = _lib.KX_F64 if self.dtype == np.float64 else _lib.KX_F32

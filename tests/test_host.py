@@ -65,7 +65,7 @@ def test_header_constants_match_binding():
     from pylops_amd import _lib
     text = open(HEADER).read()
     for name in ("KX_ABI_VERSION", "KX_AUX_BYTES", "KX_F32", "KX_F64", "KX_F16", "KX_ORDER_AUTO",
-                 "KX_ORDER_NATURAL", "KX_ORDER_ROTATED", "KX_ORDER_GROUPED", "KX_SIDE_AUTO", "KX_SIDE_RECEIVERS",
+                 "KX_ORDER_NATURAL", "KX_ORDER_ROTATED", "KX_SIDE_AUTO", "KX_SIDE_RECEIVERS",
                  "KX_SIDE_SOURCES",
                  "KX_EINVAL", "KX_ESHAPE", "KX_ETYPE", "KX_EABI", "KX_ELDS"):
         m = re.search(rf"#define {name} (-?\d+)", text)

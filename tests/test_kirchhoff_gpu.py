@@ -109,6 +109,32 @@ def test_attribute_reassignment_rebuilds_tables():
     d1 = Op @ m
     assert G.relerr(d1, orc.matvec(m)) < 1e-10
     assert G.relerr(d0, d1) > 1e-3
+    # an edit in place could not reach the device tables: it raises
+    with pytest.raises(ValueError, match="read-only"):
+        Op.trav_srcs[:, 0] += 0.1
+    # float64 LDS atomics add in a launch-dependent order: equal to rounding
+    assert G.relerr(Op @ m, d1) < 1e-13
+
+
+@pytest.mark.parametrize("scale", [1e-15, 1e15])
+def test_weights_in_extreme_units(scale):
+    """Amplitude tables in tiny or huge units (weights of 1e-30 or 1e30):
+    the float32 forward's fixed-point scale would leave the float range
+    (m 2^e overflows, or the small values underflow), so the launch
+    accumulates in float64; the result still matches the oracle."""
+    import pylops_amd
+    ts, tr, ds, dr, _, _ = pylops_amd.Kirchhoff._traveltime_table(G.z, G.x, G.s2d, G.r2d,
+                                                                  G.v0, mode="analytic")
+    amp = (scale / np.maximum(ds, 1.0), scale / np.maximum(dr, 1.0))
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav.astype(np.float32),
+              wavcenter=G.wavc, mode="byot", trav=(ts, tr), amp=amp, dynamic=True,
+              angleaperture=70, dtype="float32")
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    m = np.random.default_rng(8).standard_normal(Op.shape[1]).astype(np.float32)
+    got = np.asarray(Op @ m)
+    assert np.all(np.isfinite(got))
+    assert G.relerr(got, orc.matvec(m)) < 1e-5
 
 
 def _config1(dtype="float32", tabledtype=None):
@@ -250,7 +276,7 @@ def test_spread_lane_orders_and_receiver_blocks(dynamic):
     m = np.random.default_rng(9).standard_normal(Op.shape[1]).astype(np.float32)
     want = orc.spread(m)
     outs = []
-    for order in (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED, _lib.KX_ORDER_GROUPED):
+    for order in (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED):
         for rb in (1, 2, 4, 8, 16):
             Op.plan.sorder = order
             Op.plan.rblock = rb
@@ -306,7 +332,7 @@ def test_spread_source_blocks_bitwise(case):
     want = np.asarray(Op.spread(m))
     if case not in ("nonfinite", "kin2d_f16", "dynamic_f16_neg"):  # float16: operator tests
         assert G.relerr(want, orc.spread(m)) < 1e-5
-    for order in (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED, _lib.KX_ORDER_GROUPED):
+    for order in (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED):
         for rb in (1, 2, 4, 8, 16):
             Op.plan.sside = _lib.KX_SIDE_SOURCES
             Op.plan.sorder = order
@@ -381,3 +407,54 @@ def test_gather_samples_outside_the_staged_window(split):
     Op.plan.gsplit = split
     d = np.random.default_rng(6).standard_normal(Op.shape[0]).astype(np.float32)
     assert G.relerr(np.asarray(Op.gather(d)), orc.gather(d)) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["kin2d", "dyn2d", "kin3d", "f16", "nonfinite", "neg_byot"])
+def test_forward_convolves_in_the_spread_launch(case):
+    """kx_forward of a float32 plan convolves the workgroups' LDS traces in
+    the spread launch; its data are bitwise kx_spread followed by kx_conv,
+    for every receiver block, lane order and the float64 fallback of a
+    non-finite model (nt = 120: the padded float rows and the 41-tap filter
+    fit the LDS of the 64-bit sums; at the reference geometry's nt = 50
+    they do not, and the golden tests run the two-launch path)."""
+    import torch
+    from pylops_amd import _lib
+    t = np.arange(120) * G.PAR["dt"]
+    kw = dict(z=G.z, x=G.x, t=t, srcs=G.s2d, recs=G.r2d, vel=G.v0,
+              wav=G.wav.astype(np.float32), wavcenter=G.wavc, mode="analytic",
+              dtype="float32")
+    if case == "dyn2d":
+        kw.update(dynamic=True, angleaperture=60)
+    elif case == "kin3d":
+        kw.update(y=G.y, srcs=G.s3d_par, recs=G.r3d_par)
+    elif case == "f16":
+        kw.update(tabledtype="float16")
+    elif case == "neg_byot":
+        import pylops_amd
+        ts, tr, _, _, _, _ = pylops_amd.Kirchhoff._traveltime_table(G.z, G.x, G.s2d, G.r2d,
+                                                                    G.v0, mode="analytic")
+        kw.update(mode="byot", trav=(ts - 0.01113, tr - 0.00871), vel=None)
+    if case == "kin3d":
+        kw["t"] = G.t  # nt = 50: the two-launch path
+    Op = _K(**kw)
+    dev = torch.device("cuda", 0)
+    m = np.random.default_rng(4).standard_normal(Op.shape[1]).astype(np.float32)
+    if case == "nonfinite":
+        m[17] = np.inf
+    L = _lib.load()
+    st = _lib.stream_ptr(dev)
+    for order in (_lib.KX_ORDER_NATURAL, _lib.KX_ORDER_ROTATED):
+        for rb in (1, 2, 4, 8):
+            Op.plan.sorder, Op.plan.rblock = order, rb
+            fused = np.asarray(Op @ m)
+            traces = torch.as_tensor(np.asarray(Op.spread(m)), device=dev)
+            two = torch.empty_like(traces)
+            _lib.check(L.kx_conv(_lib.ptr(traces), _lib.ptr(two), Op.nsnr, Op.nt, Op.plan.h,
+                                 Op.plan.nh, _lib.KX_F32, st), "kx_conv")
+            two = two.cpu().numpy()
+            if case == "nonfinite":
+                np.testing.assert_array_equal(np.isnan(fused), np.isnan(two))
+                ok = np.isfinite(two)
+                np.testing.assert_array_equal(fused[ok], two[ok])
+            else:
+                np.testing.assert_array_equal(fused, two, err_msg=f"order={order} rb={rb}")
