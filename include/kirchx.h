@@ -8,7 +8,8 @@
  * an argument error (KX_EINVAL ...) and a positive value is a hipError_t.
  * kx_last_error() gives a message for the calling thread's last failure.
  * The library allocates no device memory; scratch space is a caller-owned
- * workspace whose size kx_workspace_bytes() reports.
+ * workspace whose size kx_workspace_bytes() reports (kx_forward_noalloc
+ * alone uses a KX_AUX_BYTES device array of the library instead).
  *
  * Reference interfaces replaced (PyLops, pylops/waveeqprocessing/kirchhoff.py
  * = K, pylops/signalprocessing/convolve1d.py = C1D,
@@ -196,6 +197,17 @@ int kx_xcorr_rows(const void *a, int64_t astride, int32_t la, const void *b, int
  * kx_spread followed by kx_conv) and leaves work untouched. */
 int kx_forward(const kx_plan *p, const void *m, void *d, void *work, void *stream);
 int kx_adjoint(const kx_plan *p, const void *d, void *m, void *work, void *stream);
+
+/* kx_forward with no caller memory beyond m and d, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The forward's only
+ * scratch -- the per-launch fixed-point scale and partial maxima,
+ * KX_AUX_BYTES -- lives in a device array of the library, one per GPU, held
+ * until the process ends; plan.aux is not used, and calls that overlap on
+ * one GPU race for the array.  Float32 plans whose forward convolves in the
+ * spread launch (see kx_forward) only; others fail with KX_ETYPE (float64)
+ * or KX_ELDS.  The data are bitwise those of kx_forward. */
+int kx_forward_noalloc(const kx_plan *p, const void *m, void *d, void *stream);
 
 /* ---- vector algebra for the solvers (float64 accumulation) ---- */
 /* Bytes of workspace for kx_dot. */

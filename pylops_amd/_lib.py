@@ -28,7 +28,7 @@ EXPORTS = (
     "kx_batch_workspace_bytes", "kx_batch_forward", "kx_batch_adjoint",
     "kx_seg_workspace_bytes", "kx_seg_vec", "kx_seg_dot", "kx_cgls_scalar_batch",
     "kx_eikonal", "kx_conv_rows", "kx_xcorr_rows", "kx_spread_models",
-    "kx_gather_models",
+    "kx_gather_models", "kx_forward_noalloc",
 )
 
 # solver scalar layouts and vector ops (include/kirchx.h)
@@ -197,6 +197,7 @@ def _declare(L):
         "kx_xcorr_rows": ([P, I64, I32, P, I64, I32, P, I64, I32, I32, I32, P], ctypes.c_int),
         "kx_forward": ([PP, P, P, P, P], ctypes.c_int),
         "kx_adjoint": ([PP, P, P, P, P], ctypes.c_int),
+        "kx_forward_noalloc": ([PP, P, P, P], ctypes.c_int),
         "kx_reduce_workspace_bytes": ([], I64),
         "kx_dot": ([P, P, I64, I32, P, P, P], ctypes.c_int),
         "kx_solver_workspace_bytes": ([], I64),

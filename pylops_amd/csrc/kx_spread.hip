@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "kx_internal.h"
 #include "kx_device.h"
@@ -365,4 +366,77 @@ extern "C" int kx_spread_models(const kx_plan *p, const void *m, void *traces, i
                                      : launch_models<float, true, false>(p, a, mf, yf, nmodel, st))
                       : (p->neg_trav ? launch_models<float, false, true>(p, a, mf, yf, nmodel, st)
                                      : launch_models<float, false, false>(p, a, mf, yf, nmodel, st));
+}
+
+// ---------------------------------------------------------------------------
+// the forward operator without caller memory
+// ---------------------------------------------------------------------------
+// The one-launch forward keeps only its accumulation scale and the partial
+// maxima of |m| in scratch memory (plan.aux for kx_forward).  For callers
+// that cannot allocate, that scratch is this device array: every GPU the
+// library runs on holds its own copy (KX_AUX_BYTES) until the process ends.
+static __device__ __align__(256) unsigned char scratch[KX_AUX_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The address of scratch on the current device, looked up once per device
+// (hipGetSymbolAddress with that device current).
+int scratch_address(void **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return kx_failf((int)e, "forward_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return kx_failf(KX_EINVAL, "forward_noalloc: device %d beyond the %d supported", dev,
+                        kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return kx_failf((int)e, "forward_noalloc: hipGetSymbolAddress(scratch): %s",
+                            hipGetErrorString(e));
+        if (ptr == nullptr)
+            return kx_failf(KX_EINVAL, "forward_noalloc: scratch has no device address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return KX_OK;
+}
+
+}  // namespace
+
+extern "C" int kx_forward_noalloc(const kx_plan *p, const void *m, void *d, void *stream)
+{
+    int rc = kx_plan_check(p);
+    if (rc)
+        return rc;
+    if (!m || !d || !p->h || p->nh <= 0)
+        return kx_failf(KX_EINVAL, "forward_noalloc: NULL buffer or wavelet");
+    if (p->dtype != KX_F32)
+        return kx_failf(KX_ETYPE, "forward_noalloc: float32 plans only (a float64 forward "
+                                  "needs kx_forward and its workspace)");
+    void *s = nullptr;
+    rc = scratch_address(&s);
+    if (rc)
+        return rc;
+    // the plan as given, its scale kept in this device's scratch (every
+    // launch writes the scale before the spread reads it: nothing needs
+    // clearing)
+    kx_plan q = *p;
+    q.aux = s;
+    rc = kx_spread_wavelet(&q, m, d, stream);
+    if (rc == KX_ENOFUSE)
+        return kx_failf(KX_ELDS,
+                        "forward_noalloc: this plan's forward cannot convolve in the spread "
+                        "launch (nt=%d traces with %d wavelet taps do not fit a workgroup's "
+                        "LDS, or float32 atomics were requested); use kx_forward and its "
+                        "workspace",
+                        p->nt, p->nh);
+    return rc;
 }
