@@ -51,6 +51,7 @@ if ROOT not in sys.path:
 
 METRIC = "Kirchhoff fwd+adj Gsamples/s, CGLS s/solve; FP16-table vs FP32 at ≤1e-3 err"
 HBM_PEAK_GBS = 8000.0
+CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
 def ricker(t, f0):
@@ -394,6 +395,16 @@ def roofline(res, desc, ni, traffic_key):
     if traffic is not None:
         out["traffic_frac"] = round(traffic / t / 1e9 / HBM_PEAK_GBS, 4)
         out["traffic_source"] = src
+    if dom == "spread":
+        # what binds the spread: its two 64-bit LDS atomics per (source,
+        # receiver, point) triplet, as lanes per clock per CU at the
+        # published peak engine clock
+        import torch
+        cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+        atomics = 2.0 * desc["ns"] * desc["nr"] * ni
+        out["lds_atomics"] = {"per_launch": atomics, "cus": cus, "clock_ghz": CLOCK_GHZ,
+                              "lanes_per_clk_per_cu": round(atomics / t / (cus * CLOCK_GHZ * 1e9),
+                                                            3)}
     return out
 
 

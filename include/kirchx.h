@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define KX_ABI_VERSION 3
+#define KX_ABI_VERSION 4
 
 /* Bytes of the per-plan device scratch `aux` (forward accumulation scale). */
 #define KX_AUX_BYTES 4096
@@ -120,6 +120,13 @@ typedef struct kx_plan {
                        or 4; they share the staged trace windows),
                        0 = automatic */
     int32_t pad0;
+    double wtyp;  /* typical |weight| of one (point, trace) pair: 1 kinematic,
+                     mean_s |(wp_s, wq_s)| * mean_r |(wp_r, wq_r)| over the
+                     nonzero weights dynamic (0: taken as wamp).  A float32
+                     forward accumulates in float64 instead of fixed point
+                     when max|m| wamp exceeds 1024 mean|m| wtyp (mean over the
+                     nonzero m), where fixed-point rounding would no longer be
+                     small against a typical contribution */
 } kx_plan;
 
 int kx_version(void);

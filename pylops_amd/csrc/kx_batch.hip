@@ -21,7 +21,8 @@
 // Device memory (caller-owned, kx_batch_dev_bytes), 256-byte aligned parts:
 //   BatchProb   probs[nprob]       geometry, tables, offsets into m and d
 //   SpreadScale scales[nprob]      fixed-point scale of each problem's forward
-//   float       part[nprob][kBatchScaleBlocks]  partial max|m|
+//   float       part[nprob][3][kBatchScaleBlocks]  partial max|m|, sum |m|,
+//                                  count of nonzero m
 //   unsigned    ctr[kCounters]     per-XCD work counters and queue offsets
 //   SpreadItem  sitems[n_spread]   the eight spread queues, one after the other
 //   GatherItem  gitems[n_gather]   the eight gather queues
@@ -89,7 +90,7 @@ struct BatchProb {
     const void *h, *hstar;
     int nh;
     int nseg;            // convolution blocks per trace
-    double wamp;
+    double wamp, wtyp;
 };
 
 struct SpreadItem {
@@ -213,10 +214,8 @@ __global__ __launch_bounds__(kBlock) void batch_absmax_kernel(const BatchProb *_
                                                               float *__restrict__ part)
 {
     const int p = blockIdx.y;
-    const float b =
-        absmax_block(m + probs[p].moff, probs[p].ni, blockIdx.x, kBatchScaleBlocks);
-    if (threadIdx.x == 0)
-        part[p * kBatchScaleBlocks + blockIdx.x] = b;
+    stats_block(m + probs[p].moff, probs[p].ni, blockIdx.x, kBatchScaleBlocks,
+                part + (size_t)p * 3 * kBatchScaleBlocks);
 }
 
 __global__ __launch_bounds__(kWave) void batch_scale_kernel(const BatchProb *__restrict__ probs,
@@ -224,9 +223,11 @@ __global__ __launch_bounds__(kWave) void batch_scale_kernel(const BatchProb *__r
                                                             SpreadScale *__restrict__ scales)
 {
     const int p = blockIdx.x;
-    const float mx = absmax_wave(part + p * kBatchScaleBlocks, kBatchScaleBlocks);
+    float mx;
+    double mean;
+    stats_wave(part + (size_t)p * 3 * kBatchScaleBlocks, kBatchScaleBlocks, mx, mean);
     if (threadIdx.x == 0)
-        scales[p] = scale_from_max(mx, probs[p].wamp);
+        scales[p] = scale_from_stats(mx, mean, probs[p].wamp, probs[p].wtyp);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,7 +499,7 @@ Layout place(int nprob, int64_t n_spread, int64_t n_gather, int64_t n_conv)
     L.scales = o;
     o = align256(o + sizeof(SpreadScale) * nprob);
     L.part = o;
-    o = align256(o + sizeof(float) * kBatchScaleBlocks * nprob);
+    o = align256(o + sizeof(float) * 3 * kBatchScaleBlocks * nprob);
     L.ctr = o;
     o = align256(o + sizeof(unsigned) * kCounters);
     L.sitems = o;
@@ -713,6 +714,7 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
         B.nh = a.nh;
         B.nseg = (a.nt + kConvOut - 1) / kConvOut;
         B.wamp = a.wamp;
+        B.wtyp = a.wtyp;
         moff += B.ni;
         doff += (int64_t)a.ns * a.nr * a.nt;
         cprefix[p + 1] = cprefix[p] + (int64_t)a.ns * a.nr * B.nseg;

@@ -201,13 +201,17 @@ __device__ __forceinline__ int64_t point_index(const KArgs &a, const TilePoint &
 // 2^32 contributions per sample).  On gfx950 LDS integer atomics sustain
 // ~1.5x the rate of float64 ones and float32 LDS atomics ~1/6 of float64.
 // A launch whose m holds inf or NaN accumulates in float64 instead, so that
-// they propagate as in the reference; double data always does.
+// they propagate as in the reference; double data always does.  So does a
+// launch whose typical contribution lies far below the largest one (see
+// scale_from_stats), where the fixed-point rounding would no longer be
+// small against it.
 constexpr int kSpreadWaves = 8;
 constexpr int kSpreadBlock = kSpreadWaves * kWave;
-constexpr int kScaleBlocks = 256;
+constexpr int kScaleBlocks = 128;
 
 // aux layout: [0] int exponent e, [1] int 1 = fixed point / 0 = float64,
-// [16, 16 + kScaleBlocks) float partial maxima
+// floats [16, 16 + 3 kScaleBlocks): partial max|m|, sum |m| and count of
+// nonzero m per block
 struct SpreadScale {
     int e;
     int fixed;
@@ -867,42 +871,73 @@ __device__ __forceinline__ void spread_models_body(const KArgs &a, const float *
     }
 }
 
-// Per-launch scale of the fixed-point accumulation: partial max|m| per block
-// (inf for a non-finite value), then one wave turns the maximum into e.
-// Block b of nb strides over m[0, n); the result is valid in thread 0.
-__device__ __forceinline__ float absmax_block(const float *__restrict__ m, int64_t n, int b, int nb)
+// Per-launch scale of the fixed-point accumulation: per block the partial
+// max|m| (inf for a non-finite value), sum of |m| over the finite values and
+// count of nonzero values, then one wave turns them into e.  Block b of nb
+// strides over m[0, n) and writes part[b], part[nb + b], part[2 nb + b].
+struct ModelStats {
+    float mx, sum, cnt;
+};
+
+__device__ __forceinline__ void stats_block(const float *__restrict__ m, int64_t n, int b, int nb,
+                                            float *__restrict__ part)
 {
-    float mx = 0.f;
+    ModelStats s{0.f, 0.f, 0.f};
     const int64_t stride = (int64_t)nb * blockDim.x;
     for (int64_t i = (int64_t)b * blockDim.x + threadIdx.x; i < n; i += stride) {
         float v = fabsf(m[i]);
-        if (!(v <= 3.402823466e38f))
+        if (!(v <= 3.402823466e38f)) {
             v = INFINITY;  // NaN or inf
-        mx = fmaxf(mx, v);
+        } else {
+            s.sum += v;
+            s.cnt += v != 0.f ? 1.f : 0.f;
+        }
+        s.mx = fmaxf(s.mx, v);
     }
-    __shared__ float wmax[kBlock / kWave];
 #pragma unroll
-    for (int o = kWave / 2; o >= 1; o >>= 1)
-        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+    for (int o = kWave / 2; o >= 1; o >>= 1) {
+        s.mx = fmaxf(s.mx, __shfl_xor(s.mx, o, kWave));
+        s.sum += __shfl_xor(s.sum, o, kWave);
+        s.cnt += __shfl_xor(s.cnt, o, kWave);
+    }
+    __shared__ ModelStats ws[kBlock / kWave];
     if ((threadIdx.x & (kWave - 1)) == 0)
-        wmax[threadIdx.x / kWave] = mx;
+        ws[threadIdx.x / kWave] = s;
     __syncthreads();
-    float r = wmax[0];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i)
-        r = fmaxf(r, wmax[i]);
-    return r;
+    if (threadIdx.x == 0) {
+        ModelStats r = ws[0];
+        for (int i = 1; i < (int)(blockDim.x / kWave); ++i) {
+            r.mx = fmaxf(r.mx, ws[i].mx);
+            r.sum += ws[i].sum;
+            r.cnt += ws[i].cnt;
+        }
+        part[b] = r.mx;
+        part[nb + b] = r.sum;
+        part[2 * nb + b] = r.cnt;
+    }
 }
 
-// Maximum of np partial maxima, by one wave; valid in every lane.
-__device__ __forceinline__ float absmax_wave(const float *__restrict__ part, int np)
+// The np partial statistics combined by one wave (sums in float64): the
+// maximum and the mean of the nonzero |m| (0 without any); valid in every
+// lane.
+__device__ __forceinline__ void stats_wave(const float *__restrict__ part, int np, float &mx,
+                                           double &mean)
 {
-    float mx = 0.f;
-    for (int i = threadIdx.x; i < np; i += kWave)
-        mx = fmaxf(mx, part[i]);
+    float x = 0.f;
+    double sum = 0.0, cnt = 0.0;
+    for (int i = threadIdx.x; i < np; i += kWave) {
+        x = fmaxf(x, part[i]);
+        sum += part[np + i];
+        cnt += part[2 * np + i];
+    }
 #pragma unroll
-    for (int o = kWave / 2; o >= 1; o >>= 1)
-        mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
-    return mx;
+    for (int o = kWave / 2; o >= 1; o >>= 1) {
+        x = fmaxf(x, __shfl_xor(x, o, kWave));
+        sum += __shfl_xor(sum, o, kWave);
+        cnt += __shfl_xor(cnt, o, kWave);
+    }
+    mx = x;
+    mean = cnt > 0.0 ? sum / cnt : 0.0;
 }
 
 // A contribution is at most |m| * wamp * 2 (the truncation path of a
@@ -910,7 +945,16 @@ __device__ __forceinline__ float absmax_wave(const float *__restrict__ part, int
 // The scaled image values m 2^e are float32: when max|m| 2^e would not be a
 // finite float, or values 2^-40 below the maximum would no longer be normal
 // floats (weights in extreme units), the launch accumulates in float64.
-__device__ __forceinline__ SpreadScale scale_from_max(float mx, double wamp)
+// A contribution then rounds by at most 2^-30 max|m| wamp, i.e. by
+// 2^-30 R of a typical one, R = max|m| wamp / (mean|m| wtyp) (mean over the
+// nonzero m; wtyp a typical weight, wamp when not given): when R exceeds
+// kRangeMax = 2^10 (a typical contribution rounding by more than ~2^-20 of
+// itself, and a few 1e-6 of the sums it makes), the launch accumulates in
+// float64 too.
+constexpr double kRangeMax = 1024.0;
+
+__device__ __forceinline__ SpreadScale scale_from_stats(float mx, double mean, double wamp,
+                                                        double wtyp)
 {
     const double bound = 2.0 * (double)mx * wamp;
     SpreadScale sc;
@@ -926,6 +970,9 @@ __device__ __forceinline__ SpreadScale scale_from_max(float mx, double wamp)
         sc.e = 30 - ex;
         const double top = ldexp((double)mx, sc.e);
         sc.fixed = (top <= 0x1p126 && ldexp(top, -40) >= 0x1p-126) ? 1 : 0;
+        const double wt = wtyp > 0.0 ? wtyp : wamp;
+        if (mean > 0.0 && (double)mx * wamp > kRangeMax * mean * wt)
+            sc.fixed = 0;
     }
     return sc;
 }
@@ -934,18 +981,18 @@ __global__ __launch_bounds__(kBlock) void absmax_partial_kernel(const float *__r
                                                                 int64_t n,
                                                                 float *__restrict__ part)
 {
-    const float b = absmax_block(m, n, blockIdx.x, gridDim.x);
-    if (threadIdx.x == 0)
-        part[blockIdx.x] = b;
+    stats_block(m, n, blockIdx.x, gridDim.x, part);
 }
 
 __global__ __launch_bounds__(kWave) void scale_final_kernel(const float *__restrict__ part, int np,
-                                                            double wamp,
+                                                            double wamp, double wtyp,
                                                             SpreadScale *__restrict__ sc)
 {
-    const float mx = absmax_wave(part, np);
+    float mx;
+    double mean;
+    stats_wave(part, np, mx, mean);
     if (threadIdx.x == 0)
-        *sc = scale_from_max(mx, wamp);
+        *sc = scale_from_stats(mx, mean, wamp, wtyp);
 }
 
 // ---------------------------------------------------------------------------

@@ -201,7 +201,8 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
         hipLaunchKernelGGL(absmax_partial_kernel, dim3(nb), dim3(kBlock), 0, st,
                            reinterpret_cast<const float *>(m), ni, part);
         KX_LAUNCHED("spread scale");
-        hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp, sc);
+        hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp,
+                           p->wtyp, sc);
         KX_LAUNCHED("spread scale");
     }
     // A workgroup blocking sources (float32 operators) runs the spread with
@@ -304,7 +305,7 @@ int launch_models(const kx_plan *p, const KArgs &a, const float *m, float *y, in
         hipLaunchKernelGGL(absmax_partial_kernel, dim3(nb), dim3(kBlock), 0, st, m + j * ni, ni,
                            part);
         hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(kWave), 0, st, part, nb, p->wamp,
-                           scs + j);
+                           p->wtyp, scs + j);
     }
     KX_LAUNCHED("spread models scale");
     int j = 0;

@@ -284,6 +284,21 @@ def tile(plan, tab, fmt):
     return out
 
 
+def weight_typical(wp, wq):
+    """mean over the table's nonzero entries of |(wp, wq)|, in float64
+    (0 when every entry is zero)."""
+    wp, wq = wp.reshape(-1), wq.reshape(-1)
+    total = torch.zeros((), dtype=torch.float64, device=wp.device)
+    count = torch.zeros((), dtype=torch.float64, device=wp.device)
+    step = 1 << 26
+    for i in range(0, wp.numel(), step):
+        a, b = wp[i:i + step].double(), wq[i:i + step].double()
+        r = torch.sqrt(a * a + b * b)
+        total += r.sum()
+        count += (r > 0).sum()
+    return float(total / count) if float(count) > 0 else 0.0
+
+
 def weight_bound(wp, wq):
     """max over the table of |(wp, wq)|, in float64 (device tensors of one
     side's weight tables in the tiled layout)."""

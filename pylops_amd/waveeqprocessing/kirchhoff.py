@@ -382,10 +382,14 @@ class Kirchhoff(LinearOperator):
             # |ps pr + qs qr| <= |(ps, qs)| |(pr, qr)|: bounds every weight
             plan.wamp = T.weight_bound(keep["wp_s"], keep["wq_s"]) * \
                 T.weight_bound(keep["wp_r"], keep["wq_r"])
+            # a typical weight: far below wamp, the forward accumulates in
+            # float64 (the fixed-point rounding follows wamp)
+            plan.wtyp = T.weight_typical(keep["wp_s"], keep["wq_s"]) * \
+                T.weight_typical(keep["wp_r"], keep["wq_r"])
             izmin = T.aperture_izmin(self.six, self.rix, self.nz, float(self.aperture[1]))
             keep["izmin"] = torch.as_tensor(izmin, device=dev)
         else:
-            plan.wamp = 1.0
+            plan.wamp = plan.wtyp = 1.0
         # forward scratch (accumulation scale), written by every forward
         keep["aux"] = torch.zeros(_lib.KX_AUX_BYTES // 4, dtype=torch.int32, device=dev)
         tdt = torch.float64 if self.dtype == np.float64 else torch.float32

@@ -137,6 +137,84 @@ def test_weights_in_extreme_units(scale):
     assert G.relerr(got, orc.matvec(m)) < 1e-5
 
 
+def _fixed_point_used(Op):
+    """Whether the operator's last float32 forward accumulated in fixed point
+    (the scale the forward wrote to its scratch: [e, fixed])."""
+    return int(Op._dev_tables["aux"][1].item()) == 1
+
+
+def _per_trace_relerr(got, want, keep):
+    """Largest relative L2 error over the traces (rows) on the samples
+    `keep` marks, among traces whose kept samples are not all zero."""
+    got = np.where(keep, np.asarray(got, np.float64), 0.0)
+    want = np.where(keep, np.asarray(want, np.float64), 0.0)
+    nw = np.linalg.norm(want, axis=-1)
+    ok = nw > 0
+    return float((np.linalg.norm(got - want, axis=-1)[ok] / nw[ok]).max())
+
+
+def test_forward_wide_dynamic_range_model():
+    """A model of 1e-5-sized values with one outlier of 1 (max/median ~1.5e5):
+    the fixed-point rounding, set by the outlier, would be ~1e-4 of every
+    sample the outlier does not reach; such a launch accumulates in float64,
+    and each trace away from the outlier's isochron matches the float64
+    oracle as the reference's float32 sums do.  The layered model of the
+    workload below keeps the fixed-point path."""
+    import pylops_amd
+    kw, _ = _config1()
+    Op = pylops_amd.Kirchhoff(**kw)
+    orc = O.KirchhoffOracle(**{**kw, "dtype": "float64"})
+    orc32 = O.KirchhoffOracle(**kw)
+    nx, nz = 101, 100
+    rng = np.random.default_rng(21)
+    m = 1e-5 * rng.standard_normal((nx, nz))
+    i0 = (50, 60)
+    m[i0] = 1.0
+    assert np.abs(m).max() / np.median(np.abs(m)) > 1e5
+    m = m.astype(np.float32).ravel()
+    got = np.asarray(Op @ m).reshape(11 * 51, -1)
+    assert not _fixed_point_used(Op)
+    want = orc.matvec(m.astype(np.float64)).reshape(11 * 51, -1)
+    ref32 = orc32.matvec(m).reshape(11 * 51, -1)
+    # samples farther from the outlier's arrival than the wavelet reaches
+    ts, tr, _, _, _, _ = pylops_amd.Kirchhoff._traveltime_table(
+        kw["z"], kw["x"], kw["srcs"], kw["recs"], kw["vel"], mode="analytic")
+    ii = i0[0] * nz + i0[1]
+    t_out = (ts[ii][:, None] + tr[ii][None, :]).reshape(-1, 1) / 0.004
+    k = np.arange(got.shape[1])[None, :]
+    keep = np.abs(k - t_out) > len(Op.cop.h) // 2 + 3
+    err = _per_trace_relerr(got, want, keep)
+    err32 = _per_trace_relerr(ref32, want, keep)
+    assert err < 1e-5, (err, err32)
+    # the layered workload model stays in fixed point
+    refl = 0.01 * rng.standard_normal((nx, nz))
+    refl[:, 40] += 0.7
+    Op @ refl.astype(np.float32).ravel()
+    assert _fixed_point_used(Op)
+
+
+def test_forward_wide_amplitude_spread():
+    """Dynamic byot tables whose amplitudes fall by 1e4 with depth on both
+    sides: per trace (every trace has its own scale), the float32 forward
+    matches the float64 oracle to 1e-5 whichever accumulation the launch
+    picks."""
+    import pylops_amd
+    kw, _ = _config1()
+    ts, tr, ds, dr, _, _ = pylops_amd.Kirchhoff._traveltime_table(
+        kw["z"], kw["x"], kw["srcs"], kw["recs"], kw["vel"], mode="analytic")
+    depth = np.tile(kw["z"], len(kw["x"]))[:, None] / kw["z"][-1]
+    fade = 10.0 ** (-4.0 * depth)
+    amp = (fade / np.sqrt(ds + 10.0), fade / np.sqrt(dr + 10.0))
+    kwd = dict(kw, mode="byot", trav=(ts, tr), amp=amp, dynamic=True, angleaperture=80)
+    Op = pylops_amd.Kirchhoff(**kwd)
+    orc = O.KirchhoffOracle(**{**kwd, "dtype": "float64"})
+    m = np.random.default_rng(22).standard_normal(Op.shape[1]).astype(np.float32)
+    got = np.asarray(Op @ m).reshape(11 * 51, -1)
+    want = orc.matvec(m.astype(np.float64)).reshape(11 * 51, -1)
+    err = _per_trace_relerr(got, want, np.ones_like(got, dtype=bool))
+    assert err < 1e-5, (err, _fixed_point_used(Op))
+
+
 def _config1(dtype="float32", tabledtype=None):
     """2D constant-velocity LSM workload: grid 101 x 100 at 10 m, nt=651 at
     4 ms, 11 sources and 51 receivers on the surface, 20 Hz Ricker."""

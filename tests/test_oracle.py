@@ -273,6 +273,35 @@ def test_eikonal_constant_velocity_matches_analytic():
         np.testing.assert_array_almost_equal(T.reshape(len(src), -1).T, ana, decimal=2)
 
 
+def test_eikonal_source_cell_convention():
+    """The cells around a point start at the straight-ray time to the zero
+    contour of phi (-1 at the point's cell, +1 elsewhere), the diamond
+    through the half-way points to the neighbouring centres, and stay
+    fixed: 0 at the point's own cell (inside the contour: this build keeps
+    it at 0, where scikit-fmm's signed result would hold a negative time),
+    s hx / 2 and s hz / 2 one cell away along x and z, and s rho (1 - 1/S)
+    on a diagonal, rho the distance and S = 2 (|dx|/hx + |dz|/hz); 3D adds
+    the y term."""
+    s = 1 / 1800.0
+    hx, hz = 4.0, 3.0
+    T, _ = O.eikonal_fsm(np.full((1, 9, 9), s), [(4, 0, 4)], (hx, 1.0, hz))
+    t = T[0, 0]
+    assert t[4, 4] == 0.0
+    np.testing.assert_allclose([t[3, 4], t[5, 4]], s * hx / 2, rtol=1e-15)
+    np.testing.assert_allclose([t[4, 3], t[4, 5]], s * hz / 2, rtol=1e-15)
+    rho = np.hypot(hx, hz)
+    np.testing.assert_allclose([t[3, 3], t[5, 5], t[3, 5], t[5, 3]], s * rho * 0.75, rtol=1e-15)
+    rho2 = np.hypot(2 * hx, hz)
+    np.testing.assert_allclose(t[6, 5], s * rho2 * (1 - 1 / 6.0), rtol=1e-15)
+    hy = 5.0
+    T3, _ = O.eikonal_fsm(np.full((5, 5, 5), s), [(2, 2, 2)], (hx, hy, hz))
+    t3 = T3[0]
+    assert t3[2, 2, 2] == 0.0
+    np.testing.assert_allclose(t3[3, 2, 2], s * hy / 2, rtol=1e-15)
+    rho3 = np.sqrt(hx * hx + hy * hy + hz * hz)
+    np.testing.assert_allclose(t3[3, 3, 3], s * rho3 * (1 - 1 / 6.0), rtol=1e-15)
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_eikonal_symmetry(order):
     """Constant velocity, source at the centre of a symmetric grid: every
