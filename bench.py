@@ -600,7 +600,7 @@ def time_batch(dev, steps=5, warmup=2, niter=20, sequential=True):
     t0 = time.perf_counter()
     ops = [pylops_amd.Kirchhoff(**kw, tile=tuned["tile"]) for kw in kws]
     B = KirchhoffBatch(ops, rblock=tuned["rblock"], sorder=tuned["sorder"],
-                       gsplit=tuned["gsplit"])
+                       gsplit=tuned["gsplit"], gtiles=tuned["gtiles"])
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     m = torch.as_tensor(np.concatenate([r.ravel() for _, r in probs]), device=dev)
@@ -627,12 +627,13 @@ def time_batch(dev, steps=5, warmup=2, niter=20, sequential=True):
     t_adj = float(np.mean([e[1].elapsed_time(e[2]) for e in evs])) * 1e-3
     trip = float(sum(op.ni * op.ns * op.nr for op in ops))
     res = {"workload": "lsm_batch_64_2d_uneven", "nprob": len(ops), "tables": "f16",
-           "rblock": B.rblock, "gsplit": B.gsplit, "sorder": B.sorder,
+           "rblock": B.rblock, "gsplit": B.gsplit, "gtiles": B.gtiles, "sorder": B.sorder,
            "spread_items": int(B._batch.n_spread), "gather_items": int(B._batch.n_gather),
            "model_points": int(B.shape[1]), "data_samples": int(B.shape[0]),
            "triplets": trip, "setup_s": round(t_setup, 3),
            "autotune": {"tile": list(tuned["tile"]), "rblock": tuned["rblock"],
                         "sorder": tuned["sorder"], "gsplit": tuned["gsplit"],
+                        "gtiles": tuned["gtiles"],
                         "ms_fwd_adj": tuned["ms"], "tuning_s": round(t_tune, 2)},
            "ms_per_step": round(wall * 1e3, 4),
            "fwd_ms": round(t_fwd * 1e3, 4), "adj_ms": round(t_adj * 1e3, 4),
@@ -654,15 +655,15 @@ def time_batch(dev, steps=5, warmup=2, niter=20, sequential=True):
             cgls(op, yp, niter=niter, tol=0.0, graph=True)
         torch.cuda.synchronize()
         t_seq = time.perf_counter() - t0
-        # the single-problem adjoint picks its own split of a tile's sources
-        # over waves; with the batch's split it sums in the same order, and
-        # the solutions are bitwise those of the batch
+        # the single-problem adjoint picks its own tile groups and split of
+        # a group's sources over waves; with the batch's it sums in the same
+        # order, and the solutions are bitwise those of the batch
         same = True
         for p in range(4):
-            ops[p].gsplit = B.gsplit
+            ops[p].gsplit, ops[p].gtiles = B.gsplit, B.gtiles
             xs = cgls(ops[p], ys[p], niter=niter, tol=0.0, graph=True)[0]
             same &= bool(torch.equal(xs.reshape(-1), sol[p][0].reshape(-1)))
-            ops[p].gsplit = 0
+            ops[p].gsplit = ops[p].gtiles = 0
         res.update({"cgls_s_sequential_solves": round(t_seq, 4),
                     "speedup_vs_sequential": round(t_seq / t_batch, 3),
                     "bitwise_equal_to_sequential_same_split": same})

@@ -332,7 +332,7 @@ typedef struct kx_batch {
     int32_t sorder;            /* forward lane order (all problems) */
     int32_t gsplit;            /* adjoint waves per tile (all problems) */
     int32_t nt_max, nh_max;
-    int32_t pad0;
+    int32_t gtiles;            /* adjoint tiles per wave, 1 or 2 (all problems) */
     int64_t model_len;         /* elements of the concatenated models */
     int64_t data_len;          /* elements of the concatenated data */
     int64_t n_spread, n_gather, n_conv; /* work items */
@@ -341,8 +341,9 @@ typedef struct kx_batch {
 } kx_batch;
 
 /* Device bytes a batch of these plans needs (descriptors, work lists,
- * counters).  rblock, sorder and gsplit of plans[0] apply to the whole batch
- * (0 = automatic), the other plans' values are ignored. */
+ * counters).  rblock, sorder, gsplit and gtiles of plans[0] apply to the
+ * whole batch (0 = automatic; gtiles above 2 counts as 2), the other plans'
+ * values are ignored. */
 int64_t kx_batch_dev_bytes(const kx_plan *plans, int32_t nprob);
 /* Fills *b and writes the descriptors and work lists to dev; returns after
  * the copies are done (not capturable).  The plans' tables and wavelets must

@@ -147,7 +147,7 @@ class KxBatch(ctypes.Structure):
         ("gsplit", ctypes.c_int32),
         ("nt_max", ctypes.c_int32),
         ("nh_max", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("gtiles", ctypes.c_int32),
         ("model_len", ctypes.c_int64),
         ("data_len", ctypes.c_int64),
         ("n_spread", ctypes.c_int64),

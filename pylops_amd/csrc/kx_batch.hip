@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kSpreadBlock) void batch_spread_kernel(
     launch_done(ctr);
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP>
 __global__ __launch_bounds__(kGatherBlock) void batch_gather_kernel(
     const BatchProb *__restrict__ probs, const GatherItem *__restrict__ items, unsigned *ctr,
     const int *__restrict__ qoff, const TD *__restrict__ d, TD *__restrict__ m)
@@ -178,8 +178,8 @@ __global__ __launch_bounds__(kGatherBlock) void batch_gather_kernel(
             break;
         const GatherItem w = items[it];
         const KArgs a = probs[w.p].a;
-        gather_tiles<TD, TT, DYN, NEG, JW, P, 1>(a, d + probs[w.p].doff, m + probs[w.p].moff,
-                                                 w.L);
+        gather_tiles<TD, TT, DYN, NEG, JW, P, NP>(a, d + probs[w.p].doff, m + probs[w.p].moff,
+                                                  w.L);
     }
     launch_done(ctr);
 }
@@ -237,8 +237,20 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 int dsize(int dtype) { return dtype == KX_F64 ? 8 : 4; }
 
 struct Choice {
-    int rb, rot, P;
+    int rb, rot, P, np;
 };
+
+// Adjoint tile groups of NP tiles over all problems.
+int64_t batch_groups(const kx_plan *plans, int nprob, int np)
+{
+    int64_t g = 0;
+    for (int p = 0; p < nprob; ++p) {
+        KArgs a = make_args(&plans[p]);
+        set_tile_groups(a, np);
+        g += a.ngroups;
+    }
+    return g;
+}
 
 struct Layout {
     size_t probs, scales, part, ctr, sitems, gitems, cprefix, total;
@@ -290,17 +302,30 @@ const void *bspread_fn(int rb)
     }
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int JW>
-const void *bgather_fn(int P)
+// sources per wave and pass of the batch adjoint, as for one problem
+template <bool DYN, int NP> struct BatchJW {
+    static constexpr int v = (DYN && NP == 1) ? 8 : 4;
+};
+int batch_jw(bool dyn, int np) { return (dyn && np == 1) ? 8 : 4; }
+
+template <typename TD, typename TT, bool DYN, bool NEG, int NP>
+const void *bgather_fn_np(int P)
 {
+    constexpr int JW = BatchJW<DYN, NP>::v;
     switch (P) {
     case 1:
-        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 1>);
+        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 1, NP>);
     case 2:
-        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 2>);
+        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 2, NP>);
     default:
-        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 4>);
+        return reinterpret_cast<const void *>(&batch_gather_kernel<TD, TT, DYN, NEG, JW, 4, NP>);
     }
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG>
+const void *bgather_fn(int P, int np)
+{
+    return np == 2 ? bgather_fn_np<TD, TT, DYN, NEG, 2>(P) : bgather_fn_np<TD, TT, DYN, NEG, 1>(P);
 }
 
 // Resident workgroups of a kernel per CU; without a device, the LDS and
@@ -322,7 +347,7 @@ size_t spread_lds(int rb, int nt) { return (size_t)rb * nt * 8; }
 // Kernel entry points of a batch, by its element and table types.
 struct Fns {
     const void *(*spread)(int rb, bool rot, bool dyn, bool neg);
-    const void *(*gather)(int P, bool dyn, bool neg);
+    const void *(*gather)(int P, int np, bool dyn, bool neg);
 };
 
 template <typename TD, typename TT>
@@ -340,11 +365,11 @@ const void *spread_of(int rb, bool rot, bool dyn, bool neg)
 }
 
 template <typename TD, typename TT>
-const void *gather_of(int P, bool dyn, bool neg)
+const void *gather_of(int P, int np, bool dyn, bool neg)
 {
     if (dyn)
-        return neg ? bgather_fn<TD, TT, true, true, 8>(P) : bgather_fn<TD, TT, true, false, 8>(P);
-    return neg ? bgather_fn<TD, TT, false, true, 4>(P) : bgather_fn<TD, TT, false, false, 4>(P);
+        return neg ? bgather_fn<TD, TT, true, true>(P, np) : bgather_fn<TD, TT, true, false>(P, np);
+    return neg ? bgather_fn<TD, TT, false, true>(P, np) : bgather_fn<TD, TT, false, false>(P, np);
 }
 
 Fns fns_of(int dtype, int tab_fmt)
@@ -473,14 +498,20 @@ Choice choose(const kx_plan *plans, int nprob)
     }
     if (c.rb == 1)
         c.rot = 0;
-    // adjoint waves per tile: as for one problem, on the batch's tile count
+    // adjoint tiles per wave (1 or 2) and waves per tile group: as for one
+    // problem, on the batch's group count
+    const int64_t fill = (int64_t)kGatherFill * device_cus();
+    if (q.gtiles > 0)
+        c.np = q.gtiles >= 2 ? 2 : 1;
+    else
+        c.np = batch_groups(plans, nprob, 2) * kGatherWaves >= fill ? 2 : 1;
+    const int64_t groups = c.np == 1 ? tiles : batch_groups(plans, nprob, c.np);
     c.P = 1;
     if (q.gsplit > 0) {
         while (c.P * 2 <= std::min(q.gsplit, kGatherWaves))
             c.P *= 2;
     } else {
-        const int cus = device_cus();
-        while (c.P < kGatherWaves && tiles * c.P < (int64_t)kGatherFill * cus)
+        while (c.P < kGatherWaves && groups * c.P < fill)
             c.P *= 2;
     }
     return c;
@@ -518,8 +549,10 @@ Layout layout(const kx_plan *plans, int nprob, const Choice &c)
     const int NT = kGatherWaves / c.P;
     for (int p = 0; p < nprob; ++p) {
         const kx_plan &a = plans[p];
+        KArgs ka = make_args(&a);
+        set_tile_groups(ka, c.np);
         ns += (int64_t)a.ns * ((a.nr + c.rb - 1) / c.rb);
-        ng += (a.ntiles + NT - 1) / NT;
+        ng += (ka.ngroups + NT - 1) / NT;
         nc += (int64_t)a.ns * a.nr * ((a.nt + kConvOut - 1) / kConvOut);
     }
     return place(nprob, ns, ng, nc);
@@ -613,10 +646,11 @@ int launch_bspread_d(const kx_batch *b, const View &v, const void *m, void *y, h
                : launch_bspread_rb<TD, TT, false, false, false>(b, v, m, y, st);
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P>
+template <typename TD, typename TT, bool DYN, bool NEG, int P, int NP>
 int launch_bgather(const kx_batch *b, const View &v, const void *d, void *m, hipStream_t st)
 {
-    hipLaunchKernelGGL((batch_gather_kernel<TD, TT, DYN, NEG, JW, P>), dim3(b->grid_gather),
+    constexpr int JW = BatchJW<DYN, NP>::v;
+    hipLaunchKernelGGL((batch_gather_kernel<TD, TT, DYN, NEG, JW, P, NP>), dim3(b->grid_gather),
                        dim3(kGatherBlock), 0, st, v.probs, v.gitems, v.ctr + kGatherQ,
                        v.ctr_const + kQoff + 16, reinterpret_cast<const TD *>(d),
                        reinterpret_cast<TD *>(m));
@@ -624,18 +658,24 @@ int launch_bgather(const kx_batch *b, const View &v, const void *d, void *m, hip
     return KX_OK;
 }
 
+template <typename TD, typename TT, bool DYN, bool NEG, int NP>
+int launch_bgather_np(const kx_batch *b, const View &v, const void *d, void *m, hipStream_t st)
+{
+    switch (b->gsplit) {
+    case 1:
+        return launch_bgather<TD, TT, DYN, NEG, 1, NP>(b, v, d, m, st);
+    case 2:
+        return launch_bgather<TD, TT, DYN, NEG, 2, NP>(b, v, d, m, st);
+    default:
+        return launch_bgather<TD, TT, DYN, NEG, 4, NP>(b, v, d, m, st);
+    }
+}
+
 template <typename TD, typename TT, bool DYN, bool NEG>
 int launch_bgather_p(const kx_batch *b, const View &v, const void *d, void *m, hipStream_t st)
 {
-    constexpr int JW = DYN ? 8 : 4;
-    switch (b->gsplit) {
-    case 1:
-        return launch_bgather<TD, TT, DYN, NEG, JW, 1>(b, v, d, m, st);
-    case 2:
-        return launch_bgather<TD, TT, DYN, NEG, JW, 2>(b, v, d, m, st);
-    default:
-        return launch_bgather<TD, TT, DYN, NEG, JW, 4>(b, v, d, m, st);
-    }
+    return b->gtiles == 2 ? launch_bgather_np<TD, TT, DYN, NEG, 2>(b, v, d, m, st)
+                          : launch_bgather_np<TD, TT, DYN, NEG, 1>(b, v, d, m, st);
 }
 
 template <typename TD, typename TT>
@@ -690,7 +730,7 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
     if (L.n_spread > INT32_MAX || L.n_gather > INT32_MAX || L.n_conv > INT32_MAX)
         return bfail(KX_ESHAPE, "batch: more than 2^31 work items");
     const kx_plan &q = plans[0];
-    const int JW = q.dynamic ? 8 : 4;
+    const int JW = batch_jw(q.dynamic, c.np);
     const int NT = kGatherWaves / c.P;
 
     // descriptors
@@ -706,6 +746,7 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
         BatchProb &B = probs[p];
         B.a = make_args(&a);
         B.a.rblock = c.rb;
+        set_tile_groups(B.a, c.np);
         B.moff = moff;
         B.doff = doff;
         B.ni = (int64_t)a.nx * a.ny * a.nz;
@@ -764,14 +805,15 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
     std::map<std::pair<uintptr_t, int>, int> gkeys;
     for (int p = 0; p < nprob; ++p) {
         const kx_plan &a = plans[p];
-        const int ng = (int)((a.ntiles + NT - 1) / NT);
+        const int64_t groups = probs[p].a.ngroups;  // of c.np tiles
+        const int ng = (int)((groups + NT - 1) / NT);
         // neighbouring tiles read neighbouring trace windows: runs of tile
         // groups, about one eighth of the image each
         const int span = std::max(16, (ng + kXcds - 1) / kXcds);
         for (int g = 0; g < ng; ++g) {
-            const int nti = std::min<int64_t>(NT, a.ntiles - (int64_t)g * NT);
+            const int nti = std::min<int64_t>(NT, groups - (int64_t)g * NT);
             gitems.push_back({p, g});
-            gcost.push_back((int64_t)nti * a.ns * a.nr);
+            gcost.push_back((int64_t)nti * c.np * a.ns * a.nr);
             const auto key = std::make_pair(reinterpret_cast<uintptr_t>(a.off_s), g / span);
             auto it = gkeys.emplace(key, (int)gkeys.size()).first;
             ggroup.push_back(it->second);
@@ -797,7 +839,7 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
     const int cus = device_cus();
     const int sp_cu = per_cu(f.spread(c.rb, c.rot, q.dynamic, q.neg_trav), kSpreadBlock,
                              spread_lds(c.rb, nt_max));
-    const int ga_cu = per_cu(f.gather(c.P, q.dynamic, q.neg_trav), kGatherBlock, 0);
+    const int ga_cu = per_cu(f.gather(c.P, c.np, q.dynamic, q.neg_trav), kGatherBlock, 0);
 
     b->abi_version = KX_ABI_VERSION;
     b->nprob = nprob;
@@ -810,7 +852,7 @@ extern "C" int kx_batch_build(kx_batch *b, const kx_plan *plans, int32_t nprob, 
     b->gsplit = c.P;
     b->nt_max = nt_max;
     b->nh_max = nh_max;
-    b->pad0 = 0;
+    b->gtiles = c.np;
     b->model_len = moff;
     b->data_len = doff;
     b->n_spread = L.n_spread;

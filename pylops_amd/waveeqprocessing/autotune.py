@@ -191,13 +191,13 @@ def autotune(reps=3, tiles=None, cache=None, **kw):
 def autotune_batch(kws, reps=3, tiles=None, cache=None):
     """Fastest launch shapes for ``KirchhoffBatch([Kirchhoff(**kw) for kw
     in kws])``: returns ``tile`` (for every problem), ``rblock``,
-    ``sorder``, ``gsplit`` (for the batch) and the measured ``ms`` (batched
-    forward + batched adjoint)."""
+    ``sorder``, ``gsplit``, ``gtiles`` (for the batch) and the measured
+    ``ms`` (batched forward + batched adjoint)."""
     from pylops_amd.waveeqprocessing.kirchhoff import Kirchhoff
     from pylops_amd.waveeqprocessing.kirchhoffbatch import KirchhoffBatch
     _lib.require_device()
-    kws = [{k: v for k, v in kw.items() if k not in ("tile", "rblock", "sorder", "gsplit")}
-           for kw in kws]
+    kws = [{k: v for k, v in kw.items()
+            if k not in ("tile", "rblock", "sorder", "gsplit", "gtiles")} for kw in kws]
     device = kws[0].get("device") or torch.device("cuda", torch.cuda.current_device())
     path = None if cache is False else _cache_path(cache)
     key = _key("batch", kws, device)
@@ -205,6 +205,7 @@ def autotune_batch(kws, reps=3, tiles=None, cache=None):
         hit = _load(path).get(key)
         if hit is not None:
             hit["tile"] = tuple(hit["tile"])
+            hit.setdefault("gtiles", 0)  # entries from before gtiles was tuned
             return hit
     ndims = _geometry(kws[0])[0]
     tiles = tiles or (TILES_2D if ndims == 2 else TILES_3D)
@@ -230,11 +231,12 @@ def autotune_batch(kws, reps=3, tiles=None, cache=None):
             fwd.append((_timer(lambda: B._into(m, d, False), reps), rb, so))
         adj = []
         for gs in GSPLITS:
-            B = KirchhoffBatch(ops, gsplit=gs)
-            adj.append((_timer(lambda: B._into(d, a, True), reps), gs))
+            for gt in (1, 2):
+                B = KirchhoffBatch(ops, gsplit=gs, gtiles=gt)
+                adj.append((_timer(lambda: B._into(d, a, True), reps), gs, gt))
         tf, rb, so = min(fwd)
-        ta, gs = min(adj)
-        cand = {"tile": tuple(tile), "rblock": rb, "sorder": so, "gsplit": gs,
+        ta, gs, gt = min(adj)
+        cand = {"tile": tuple(tile), "rblock": rb, "sorder": so, "gsplit": gs, "gtiles": gt,
                 "ms": round(tf + ta, 4), "ms_forward": round(tf, 4), "ms_adjoint": round(ta, 4)}
         if best is None or cand["ms"] < best["ms"]:
             best = cand

@@ -35,10 +35,12 @@ class KirchhoffBatch(LinearOperator):
     sorder : int
         Forward lane order, ``_lib.KX_ORDER_*`` (0 = automatic).
     gsplit : int
-        Adjoint waves per tile, 1, 2 or 4 (0 = automatic).
+        Adjoint waves per tile group, 1, 2 or 4 (0 = automatic).
+    gtiles : int
+        Adjoint tiles per wave, 1 or 2 (0 = automatic).
     """
 
-    def __init__(self, ops, rblock=0, sorder=0, gsplit=0, name="B"):
+    def __init__(self, ops, rblock=0, sorder=0, gsplit=0, gtiles=0, name="B"):
         ops = list(ops)
         if not ops:
             raise ValueError("KirchhoffBatch needs at least one operator")
@@ -62,6 +64,7 @@ class KirchhoffBatch(LinearOperator):
         plans[0].rblock = int(rblock)
         plans[0].sorder = int(sorder)
         plans[0].gsplit = int(gsplit)
+        plans[0].gtiles = int(gtiles)
         L = _lib.load()
         nbytes = L.kx_batch_dev_bytes(plans, n)
         if nbytes < 0:
@@ -91,6 +94,10 @@ class KirchhoffBatch(LinearOperator):
     @property
     def gsplit(self):
         return self._batch.gsplit
+
+    @property
+    def gtiles(self):
+        return self._batch.gtiles
 
     @property
     def sorder(self):

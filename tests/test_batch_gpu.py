@@ -77,6 +77,23 @@ def test_batch_products_match_single_operators(dynamic, dtype, tab):
             assert G.relerr(ab[mo[p]:mo[p + 1]], a1) < 1e-12
 
 
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("gsplit,gtiles", [(1, 2), (4, 2), (2, 1)])
+def test_batch_adjoint_bitwise_with_matched_shape(dynamic, gsplit, gtiles):
+    """The batch adjoint runs the single-problem adjoint body on each
+    problem: with the same tile groups and split, each problem's image is
+    bitwise its own operator's."""
+    from pylops_amd.waveeqprocessing import KirchhoffBatch
+    _, ops = _ops(5, 17, dynamic, "float32")
+    B = KirchhoffBatch(ops, gsplit=gsplit, gtiles=gtiles)
+    d = np.random.default_rng(9).standard_normal(B.shape[0]).astype(np.float32)
+    ab = B.H @ d
+    mo, do = B.model_offsets, B.data_offsets
+    for p, op in enumerate(ops):
+        op.gsplit, op.gtiles = B.gsplit, B.gtiles
+        np.testing.assert_array_equal(ab[mo[p]:mo[p + 1]], op.H @ d[do[p]:do[p + 1]])
+
+
 def test_batch_products_vs_oracle():
     from pylops_amd.waveeqprocessing import KirchhoffBatch
     kws, ops = _ops(5, 7, True, "float32")
@@ -92,18 +109,21 @@ def test_batch_products_vs_oracle():
         assert G.relerr(ab[mo[p]:mo[p + 1]], orc.rmatvec(d[do[p]:do[p + 1]])) < 1e-5
 
 
-@pytest.mark.parametrize("rblock,sorder,gsplit", [(0, 0, 0), (1, 1, 1), (4, 2, 2), (8, 1, 4)])
-def test_batch_launch_shapes_agree(rblock, sorder, gsplit):
-    """Every receiver block, lane order and adjoint split gives the same
+@pytest.mark.parametrize("rblock,sorder,gsplit,gtiles",
+                         [(0, 0, 0, 0), (1, 1, 1, 1), (4, 2, 2, 2), (8, 1, 4, 1), (2, 2, 4, 2)])
+def test_batch_launch_shapes_agree(rblock, sorder, gsplit, gtiles):
+    """Every receiver block, lane order and adjoint shape gives the same
     forward (exactly, fixed point) and the same adjoint (to rounding)."""
     from pylops_amd.waveeqprocessing import KirchhoffBatch
     _, ops = _ops(6, 21, False, "float32")
     B0 = KirchhoffBatch(ops)
-    B = KirchhoffBatch(ops, rblock=rblock, sorder=sorder, gsplit=gsplit)
+    B = KirchhoffBatch(ops, rblock=rblock, sorder=sorder, gsplit=gsplit, gtiles=gtiles)
     if rblock:
         assert B.rblock == rblock
     if gsplit:
         assert B.gsplit == gsplit
+    if gtiles:
+        assert B.gtiles == gtiles
     rng = np.random.default_rng(8)
     m = rng.standard_normal(B.shape[1]).astype(np.float32)
     d = rng.standard_normal(B.shape[0]).astype(np.float32)
@@ -251,8 +271,10 @@ def test_autotune_batch(tmp_path):
     kws = [_problem(rng, False, "float32", "float16") for _ in range(4)]
     best = autotune_batch(kws, reps=1, cache=False)
     ops = [pylops_amd.Kirchhoff(**kw, tile=best["tile"]) for kw in kws]
-    B = KirchhoffBatch(ops, rblock=best["rblock"], sorder=best["sorder"], gsplit=best["gsplit"])
+    B = KirchhoffBatch(ops, rblock=best["rblock"], sorder=best["sorder"], gsplit=best["gsplit"],
+                       gtiles=best["gtiles"])
     assert B.rblock == best["rblock"] and B.gsplit == best["gsplit"]
+    assert B.gtiles == best["gtiles"]
     B0 = KirchhoffBatch([pylops_amd.Kirchhoff(**kw) for kw in kws])
     m = rng.standard_normal(B.shape[1]).astype(np.float32)
     d = rng.standard_normal(B.shape[0]).astype(np.float32)

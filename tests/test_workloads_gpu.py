@@ -250,13 +250,13 @@ def test_batch64_uneven_vs_oracle_and_cgls():
             xx = np.dot(u.astype(np.float64), (B.H @ v).astype(np.float64))
             assert abs(xx - yy) <= 1e-5 * max(abs(xx), abs(yy))
             # 20 graph-replayed CGLS iterations for all problems at once
-            # against CGLS per problem (each with the batch's adjoint split,
+            # against CGLS per problem (each with the batch's adjoint shape,
             # so that its float32 adjoint sums in the same order)
             y = torch.as_tensor(fb, device=DEV)
             sol = cgls_batch(B, y, niter=20, tol=0.0)
             ys = B.split_data(y)
             for p, op in enumerate(ops):
-                op.gsplit = B.gsplit
+                op.gsplit, op.gtiles = B.gsplit, B.gtiles
                 x1 = cgls(op, ys[p], niter=20, tol=0.0, graph=True)
                 assert sol[p][2] == 20 and x1[2] == 20
                 assert _rel(_host(sol[p][0]), _host(x1[0])) < 1e-4, p
