@@ -638,7 +638,8 @@ def time_batch(dev, steps=5, warmup=2, niter=20, sequential=True):
            "ms_per_step": round(wall * 1e3, 4),
            "fwd_ms": round(t_fwd * 1e3, 4), "adj_ms": round(t_adj * 1e3, 4),
            "fwd_adj_gsamples_s": round(2 * trip / wall / 1e9, 2),
-           "model_bytes": int(batch_model_bytes(ops))}
+           "model_bytes": int(batch_model_bytes(ops)),
+           "compulsory_bytes": int(sum(compulsory_bytes(op) for op in ops))}
     B._into(m, y, False)
     cgls_batch(B, y, niter=2, tol=0.0)
     torch.cuda.synchronize()
@@ -702,7 +703,9 @@ def batch_line(dev, args):
                             sequential=not args.no_solve)
     t_dom = max(res["fwd_ms"], res["adj_ms"]) * 1e-3
     dom = "batch_forward" if res["fwd_ms"] >= res["adj_ms"] else "batch_adjoint"
-    achieved = res["model_bytes"] / t_dom / 1e9
+    comp = res["compulsory_bytes"]
+    achieved = comp / t_dom / 1e9
+    traffic, src = measured_traffic(res["workload"], dom, "f16")
     out = {
         "metric": METRIC,
         "value": res["fwd_adj_gsamples_s"],
@@ -722,9 +725,14 @@ def batch_line(dev, args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(res["workload"], dom, "f16")[0],
-                     "model": "separable-stream", "model_bytes": res["model_bytes"]},
+                     "traffic": traffic, "model": "compulsory", "model_bytes": comp,
+                     "separable_stream_bytes": res["model_bytes"],
+                     "separable_stream_frac": round(res["model_bytes"] / t_dom / 1e9
+                                                    / HBM_PEAK_GBS, 4)},
     }
+    if traffic is not None:
+        out["roofline"]["traffic_frac"] = round(traffic / t_dom / 1e9 / HBM_PEAK_GBS, 4)
+        out["roofline"]["traffic_source"] = src
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_batch(probs)
     return out
