@@ -126,6 +126,85 @@ def test_single_table_vs_oracle(dtype, dim3):
     assert G.relerr(Op.H @ d.astype(dtype), orc.rmatvec(d.astype(dtype))) < tol
 
 
+def test_engines_and_their_errors():
+    """engine: the reference names ("numpy", "numba", "cuda") run the HIP
+    kernels (bitwise the default engine's products), an unknown name raises
+    KeyError and "cuda" with one combined table NotImplementedError, as in
+    the reference (K:1003-1046, pytests/test_kirchhoff.py:219)."""
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dtype="float32")
+    Op = _K(**kw)
+    rng = np.random.default_rng(41)
+    m = rng.standard_normal(Op.shape[1]).astype(np.float32)
+    d = rng.standard_normal(Op.shape[0]).astype(np.float32)
+    for engine in ("numpy", "numba", "cuda"):
+        Oe = _K(engine=engine, **kw)
+        assert Oe.engine == engine
+        np.testing.assert_array_equal(Oe @ m, Op @ m)
+        np.testing.assert_array_equal(Oe.H @ d, Op.H @ d)
+    with pytest.raises(KeyError):
+        _K(engine="fortran", **kw)
+    trav = (Op.trav_srcs[:, :, None] + Op.trav_recs[:, None, :]).reshape(Op.ni, -1)
+    kws = dict(kw, mode="byot", trav=trav)
+    with pytest.raises(NotImplementedError):
+        _K(engine="cuda", **kws)
+    _K(engine="numba", **kws)  # the other engines take a single table
+
+
+@pytest.mark.parametrize("dynamic,dim3", [(False, False), (True, False), (False, True)])
+def test_trav_vs_travsrcrec(dynamic, dim3):
+    """pytests/test_kirchhoff.py:320-460 restated (2D kinematic and dynamic,
+    3D kinematic, as there): the operator built with the separable tables
+    and the one given a combined table (plus the amplitude tables when
+    dynamic) with angleaperture=None agree, forward and adjoint."""
+    s, r = (G.s3d_par, G.r3d_par) if dim3 else (G.s2d, G.r2d)
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=s, recs=r, vel=G.v0, wav=G.wav, wavcenter=G.wavc,
+              mode="analytic", dynamic=dynamic, angleaperture=None)
+    if dim3:
+        kw["y"] = G.y
+    Dop = _K(**kw)
+    ni = Dop.ni
+    trav = (np.asarray(Dop.trav_srcs)[:, :, None]
+            + np.asarray(Dop.trav_recs)[:, None, :]).reshape(ni, -1)
+    amp = (np.asarray(Dop.amp_srcs), np.asarray(Dop.amp_recs)) if dynamic else None
+    D1op = _K(**dict(kw, trav=trav, amp=amp))
+    rng = np.random.default_rng(43)
+    xx = rng.normal(0, 1, Dop.shape[1])
+    yy = rng.normal(0, 1, Dop.shape[0])
+    np.testing.assert_array_almost_equal(Dop @ xx, D1op @ xx, decimal=2)
+    np.testing.assert_array_almost_equal(Dop.H @ yy, D1op.H @ yy, decimal=2)
+
+
+@pytest.mark.parametrize("dim3", [False, True])
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("nt", [50, 15])
+def test_one_source_one_receiver(dim3, dynamic, nt):
+    """The smallest geometry: one source and one receiver (one trace, one
+    receiver per workgroup, one source per adjoint pass) and a trace
+    shorter than the wavelet (nt = 15 < 21 taps), against the oracle."""
+    t = np.arange(nt) * G.PAR["dt"]
+    if dim3:
+        s = np.array([[G.y[1]], [G.x[3]], [2.0]])
+        r = np.array([[G.y[2]], [G.x[9]], [2.0]])
+    else:
+        s = np.array([[G.x[3]], [2.0]])
+        r = np.array([[G.x[9]], [2.0]])
+    kw = dict(z=G.z, x=G.x, t=t, srcs=s, recs=r, vel=G.v0, wav=G.wav.astype(np.float32),
+              wavcenter=G.wavc, mode="analytic", dynamic=dynamic, dtype="float32")
+    if dim3:
+        kw["y"] = G.y
+    if dim3 and dynamic:
+        kw["angleaperture"] = 60
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    rng = np.random.default_rng(47)
+    m = rng.standard_normal(Op.shape[1]).astype(np.float32)
+    d = rng.standard_normal(Op.shape[0]).astype(np.float32)
+    assert Op.shape == (nt, Op.ni)
+    assert G.relerr(Op @ m, orc.matvec(m)) < 1e-5
+    assert G.relerr(Op.H @ d, orc.rmatvec(d)) < 1e-5
+
+
 def test_attribute_reassignment_rebuilds_tables():
     Op = _K(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
             wavcenter=G.wavc, mode="analytic")
