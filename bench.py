@@ -210,7 +210,10 @@ def compulsory_bytes(Op, dtype_bytes=4):
     tr = 2 if Op.tabledtype == np.float16 else tb  # float16: receiver offsets only
     b = (ns * tb + nr * tr) * ntiles * 64 + (ns + nr) * ntiles * 4
     if Op.dynamic:
-        b += 2 * (ns + nr) * ntiles * 64 * dtype_bytes + Op.nsnr * 4
+        # two weight tables per side; float16 plans keep the receiver's in
+        # float16
+        b += 2 * (ns * dtype_bytes + nr * (2 if tr == 2 else dtype_bytes)) * ntiles * 64
+        b += Op.nsnr * 4
     return b + Op.ni * dtype_bytes + Op.nsnr * Op.nt * dtype_bytes
 
 

@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define KX_ABI_VERSION 4
+#define KX_ABI_VERSION 5
 
 /* Bytes of the per-plan device scratch `aux` (forward accumulation scale). */
 #define KX_AUX_BYTES 4096
@@ -82,7 +82,11 @@ extern "C" {
  *            with KX_F16 the source side off_s is f32), traveltime in
  *            samples minus base,
  *   bases    base[ntiles][n]      int32, samples,
- *   weights  wp/wq[ntiles][n][64] element type dtype (dynamic only),
+ *   weights  wp/wq[ntiles][n][64] element type dtype (dynamic only; with
+ *            KX_F16 the receiver side wp_r, wq_r is f16, the source side
+ *            f32: scale the receiver weights to at most 1 and the source
+ *            weights by the same factor, so that their products keep
+ *            their values and the f16 range fits any units),
  * n = ns for the source side and nr for the receiver side.
  * A tile is a tz*tx*ty = 64 block of image points; tiles are enumerated
  * z fastest, then x, then y; lanes inside a tile likewise. */
@@ -145,7 +149,7 @@ int kx_encode_table(const kx_plan *geom, const double *tabT, int32_t n,
                     void *stream);
 
 /* Reorders a [n][ni] float64 table into [ntiles][n][64] of dtype `fmt`
- * (KX_F32 or KX_F64), zero padded (dynamic weights). */
+ * (KX_F32, KX_F64 or KX_F16), zero padded (dynamic weights). */
 int kx_tile_table(const kx_plan *geom, const double *tabT, int32_t n,
                   int32_t fmt, void *out, void *stream);
 

@@ -12,7 +12,7 @@ import torch  # noqa: F401  (must precede the HIP library)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libkirchx.so")
 
-KX_ABI_VERSION = 4
+KX_ABI_VERSION = 5
 KX_AUX_BYTES = 4096
 KX_F32, KX_F64, KX_F16 = 0, 1, 2
 KX_ORDER_AUTO, KX_ORDER_NATURAL, KX_ORDER_ROTATED = 0, 1, 2

@@ -168,8 +168,11 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(KArgs a, const double *__r
     const int j = (int)(w / a.ntiles);
     const int t = (int)(w % a.ntiles);
     const TilePoint p = tile_point(a, t, lane);
-    out[((size_t)t * n + j) * kWave + lane] =
-        p.valid ? (TD)tabT[(size_t)j * ni + point_index(a, p)] : TD(0);
+    const double v = p.valid ? tabT[(size_t)j * ni + point_index(a, p)] : 0.0;
+    if constexpr (sizeof(TD) == 2)
+        out[((size_t)t * n + j) * kWave + lane] = __float2half_rn((float)v);
+    else
+        out[((size_t)t * n + j) * kWave + lane] = (TD)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -332,6 +335,9 @@ extern "C" int kx_tile_table(const kx_plan *geom, const double *tabT, int32_t n,
     else if (fmt == KX_F64)
         hipLaunchKernelGGL(tile_kernel<double>, dim3((int)nblk), dim3(kBlock), 0, st, a, tabT, n,
                            ni, reinterpret_cast<double *>(out));
+    else if (fmt == KX_F16)
+        hipLaunchKernelGGL(tile_kernel<__half>, dim3((int)nblk), dim3(kBlock), 0, st, a, tabT, n,
+                           ni, reinterpret_cast<__half *>(out));
     else
         return kx_fail(KX_ETYPE, "tile_table: unknown format %d", fmt);
     KX_LAUNCHED("tile_table");

@@ -65,6 +65,14 @@ template <typename T, int N> struct alignas(sizeof(T) * N) VecN {
 // one float16 rounding, the receiver's.
 template <typename TT> struct SrcTab { using type = TT; };
 template <> struct SrcTab<__half> { using type = float; };
+// Receiver-side dynamic weights: float16 beside float16 offsets (scaled to
+// at most 1 by the caller), the data type otherwise; wv reads one as a
+// float of the arithmetic.
+template <typename TD, typename TT> struct RecW { using type = TD; };
+template <> struct RecW<float, __half> { using type = __half; };
+__device__ __forceinline__ float wv(float x) { return x; }
+__device__ __forceinline__ float wv(__half x) { return __half2float(x); }
+__device__ __forceinline__ double wv(double x) { return x; }
 
 __device__ __forceinline__ float ld_off(const float *p) { return *p; }
 __device__ __forceinline__ float ld_off(const __half *p) { return __half2float(*p); }
@@ -292,7 +300,7 @@ template <typename TD, typename TT, int ORD, int RB> struct SpreadTile {
     typename SrcTab<TT>::type os;  // raw table elements: converted at use, so that a prefetch
     TT orr[RB];  // never waits for them
     TD ps, qs;
-    TD pr[RB], qr[RB];
+    typename RecW<TD, TT>::type pr[RB], qr[RB];
     int bs;
     int br[BaseLoads<ORD, RB>::n];
 };
@@ -361,8 +369,9 @@ __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c,
         if constexpr (BaseLoads<ORD, RB>::n > 1)
             L.br[r] = brow[(unsigned)(j + vz)];
         if constexpr (DYN) {
-            L.pr[r] = (reinterpret_cast<const TD *>(a.wp_r) + tr * kWave)[e];
-            L.qr[r] = (reinterpret_cast<const TD *>(a.wq_r) + tr * kWave)[e];
+            using RW = typename RecW<TD, TT>::type;
+            L.pr[r] = (reinterpret_cast<const RW *>(a.wp_r) + tr * kWave)[e];
+            L.qr[r] = (reinterpret_cast<const RW *>(a.wq_r) + tr * kWave)[e];
         }
     }
 }
@@ -478,9 +487,9 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, 
         bool on = true;
         if constexpr (DYN) {
             if constexpr (XS)  // (ps, qs) here are the receiver's, pr, qr the source's
-                w = fma(mv * cur.qr[r], cur.qs, (mv * cur.pr[r]) * cur.ps);
+                w = fma(mv * (TD)wv(cur.qr[r]), cur.qs, (mv * (TD)wv(cur.pr[r])) * cur.ps);
             else
-                w = fma(mqs, cur.qr[r], mps * cur.pr[r]);
+                w = fma(mqs, (TD)wv(cur.qr[r]), mps * (TD)wv(cur.pr[r]));
             on = iz >= izm[r];
         }
         spread_sample<TD, TC, NEG, ACC>(T, cur.bs + step_base<ORD, RB>(cur.br, r), w, on,
@@ -728,7 +737,8 @@ template <typename TT, int K> struct ModelsTile {
     float mv[K];
     typename SrcTab<TT>::type os;
     TT orr;
-    float ps, qs, pr, qr;
+    float ps, qs;
+    typename RecW<float, TT>::type pr, qr;
     int bs, br;
 };
 
@@ -754,8 +764,9 @@ __device__ __forceinline__ void models_load(const KArgs &a, const TileCursor &c,
     if constexpr (DYN) {
         L.ps = (reinterpret_cast<const float *>(a.wp_s) + ts * kWave)[(unsigned)lane];
         L.qs = (reinterpret_cast<const float *>(a.wq_s) + ts * kWave)[(unsigned)lane];
-        L.pr = (reinterpret_cast<const float *>(a.wp_r) + tr * kWave)[(unsigned)lane];
-        L.qr = (reinterpret_cast<const float *>(a.wq_r) + tr * kWave)[(unsigned)lane];
+        using RW = typename RecW<float, TT>::type;
+        L.pr = (reinterpret_cast<const RW *>(a.wp_r) + tr * kWave)[(unsigned)lane];
+        L.qr = (reinterpret_cast<const RW *>(a.wq_r) + tr * kWave)[(unsigned)lane];
     }
 }
 
@@ -797,7 +808,7 @@ __device__ __forceinline__ void models_accumulate(const ModelsTile<TT, K> &cur, 
             mv = ldexpf(mv, e[j]);
         float w = mv;
         if constexpr (DYN)
-            w = fmaf(mv * cur.qs, cur.qr, (mv * cur.ps) * cur.pr);
+            w = fmaf(mv * cur.qs, wv(cur.qr), (mv * cur.ps) * wv(cur.pr));
         if (w != 0.f) {
             const int it = j * nt + k;
             const float w1 = w * f;
@@ -1103,8 +1114,9 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
     const TT *offr = reinterpret_cast<const TT *>(a.off_r);
     const TD *wps = reinterpret_cast<const TD *>(a.wp_s);
     const TD *wqs = reinterpret_cast<const TD *>(a.wq_s);
-    const TD *wpr = reinterpret_cast<const TD *>(a.wp_r);
-    const TD *wqr = reinterpret_cast<const TD *>(a.wq_r);
+    using RW = typename RecW<TD, TT>::type;
+    const RW *wpr = reinterpret_cast<const RW *>(a.wp_r);
+    const RW *wqr = reinterpret_cast<const RW *>(a.wq_r);
     const unsigned ntm1 = (unsigned)(a.nt - 1);
     const int vz = vgpr_zero();
     TP(&w)[K][JW][kWin] = win[wave];
@@ -1177,14 +1189,14 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
             // ahead (the window positions of the round ahead depend on them)
             TT orr_c[NP];
             int br_c[NP], br_n[NP];
-            TD pr_c[NP], qr_c[NP];
+            RW pr_c[NP], qr_c[NP];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
                 const size_t ro = (size_t)t[j] * a.nr;
                 orr_c[j] = offr[ro * kWave + lane];
                 br_c[j] = __builtin_amdgcn_readfirstlane(a.base_r[ro + vz]);
                 br_n[j] = a.nr > 1 ? a.base_r[ro + 1 + vz] : 0;
-                pr_c[j] = qr_c[j] = TD(0);
+                pr_c[j] = qr_c[j] = RW(0.0f);
                 if constexpr (DYN) {
                     pr_c[j] = wpr[ro * kWave + lane];
                     qr_c[j] = wqr[ro * kWave + lane];
@@ -1214,8 +1226,8 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                 for (int j = 0; j < NP; ++j) {
                     orr[j] = (TC)ld_off(&orr_c[j]);
                     br[j] = br_c[j];
-                    pr[j] = pr_c[j];
-                    qr[j] = qr_c[j];
+                    pr[j] = (TD)wv(pr_c[j]);
+                    qr[j] = (TD)wv(qr_c[j]);
                 }
                 // next round's receiver side and samples, in flight meanwhile
                 if (r + 1 < a.nr) {

@@ -276,7 +276,7 @@ def tile(plan, tab, fmt):
     """float64 (ni, n) -> [ntiles][n][64] of dtype fmt, zero padded."""
     n = tab.shape[1]
     tabT = tab.t().contiguous()
-    tdt = torch.float64 if fmt == _lib.KX_F64 else torch.float32
+    tdt = {_lib.KX_F64: torch.float64, _lib.KX_F16: torch.float16}.get(fmt, torch.float32)
     out = torch.empty((plan.ntiles, n, 64), dtype=tdt, device=tab.device)
     rc = _lib.load().kx_tile_table(ctypes.byref(plan), _lib.ptr(tabT), n, fmt, _lib.ptr(out),
                                    _lib.stream_ptr(tab.device))

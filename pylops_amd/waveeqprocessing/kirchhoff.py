@@ -76,9 +76,11 @@ class Kirchhoff(LinearOperator):
     reference names ("numpy", "numba", "cuda"); every engine runs the HIP
     kernels.  Extra keyword-only parameters:
 
-    tabledtype : ``"float16"`` stores the receiver traveltime offsets in
-        half precision (float32 operators only; the source tables stay
-        float32); default is the operator dtype.
+    tabledtype : ``"float16"`` stores the receiver traveltime offsets and,
+        when dynamic, the receiver weights (scaled to at most 1, the source
+        weights by the same factor) in half precision (float32 operators
+        only; the source tables stay float32); default is the operator
+        dtype.
     tile : ``(tz, tx, ty)`` image tile of 64 points.
     device : torch device of the tables (default: current HIP device).
     rblock : receivers (or sources, see ``sside``) per forward workgroup
@@ -375,10 +377,18 @@ class Kirchhoff(LinearOperator):
             pr, qr = T.side_weights(T.as_f64(tab["amp_recs"], dev),
                                     T.as_f64(tab["angle_recs"], dev),
                                     amin, amax, self.aperturetap)
+            rfmt = wfmt
+            if fmt == _lib.KX_F16:
+                # float16 receiver weights, scaled to at most 1 (any units
+                # fit float16), the source weights by the same factor
+                sc = float(torch.sqrt(pr * pr + qr * qr).max())
+                if sc > 0.0:
+                    ps, qs, pr, qr = ps * sc, qs * sc, pr / sc, qr / sc
+                rfmt = _lib.KX_F16
             keep["wp_s"] = T.tile(plan, ps, wfmt)
             keep["wq_s"] = T.tile(plan, qs, wfmt)
-            keep["wp_r"] = T.tile(plan, pr, wfmt)
-            keep["wq_r"] = T.tile(plan, qr, wfmt)
+            keep["wp_r"] = T.tile(plan, pr, rfmt)
+            keep["wq_r"] = T.tile(plan, qr, rfmt)
             # |ps pr + qs qr| <= |(ps, qs)| |(pr, qr)|: bounds every weight
             plan.wamp = T.weight_bound(keep["wp_s"], keep["wq_s"]) * \
                 T.weight_bound(keep["wp_r"], keep["wq_r"])
