@@ -878,7 +878,11 @@ def precision_report(dev, which, niter):
     against the float64 operator (float64 tables, float64 arithmetic; its
     kernels are checked against the oracle to 1e-10), for the forward of the
     reflectivity, the adjoint of the float64 data and the model after
-    `niter` CGLS iterations; and the fwd+adj time of both table formats."""
+    `niter` CGLS iterations; and the wall time of forward(reflectivity) +
+    adjoint(data) with both table formats.  The 2D reflectivity is sparse
+    and the spread skips waves whose image points are all zero, so there
+    these times are shorter than the dense-image step of the headline line
+    (the 3D reflectivity carries noise everywhere and is dense)."""
     import torch
     import pylops_amd
     from pylops_amd.optimization.basic import cgls
@@ -914,8 +918,7 @@ def precision_report(dev, which, niter):
         dt = (time.perf_counter() - t0) / n
         rx = cgls(Op, d32, niter=niter, tol=0.0, graph=True)
         x, cost = rx[0].reshape(-1), np.asarray(rx[5])
-        res[tabs] = {"ms_fwd_adj": round(dt * 1e3, 3),
-                     "gsamples_s": round(2 * Op.ni * Op.ns * Op.nr / dt / 1e9, 2),
+        res[tabs] = {"ms_fwd_adj_refl": round(dt * 1e3, 3),
                      "rel_err_fwd": rel(d, d64), "rel_err_adj": rel(a, a64),
                      "rel_err_cgls_x": rel(x, x64),
                      "cgls_residual_norm_rel_diff_max": float(
@@ -924,7 +927,8 @@ def precision_report(dev, which, niter):
                      "cgls_residual_norm_reduction_fp64": float(cost64[-1] / cost64[0])}
         del Op
         torch.cuda.empty_cache()
-    res["fp16_speedup"] = round(res["f32"]["ms_fwd_adj"] / res["f16"]["ms_fwd_adj"], 3)
+    res["fp16_speedup_refl"] = round(res["f32"]["ms_fwd_adj_refl"]
+                                     / res["f16"]["ms_fwd_adj_refl"], 3)
     return res
 
 
