@@ -679,20 +679,24 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
 
     // the next tile is loaded while this one is accumulated (a twice
     // unrolled loop with two buffers and no copies measured slower: the
-    // second buffer costs registers and occupancy)
+    // second buffer costs registers and occupancy).  Loads only ever fill
+    // nxt, which is copied to cur at the top of the loop: the wait for a
+    // tile's loads then falls a whole accumulation after they were issued
+    // (loading the first tile into cur makes the compiler wait for the
+    // prefetch before every accumulation).
     SpreadTile<TD, TT, ORD, RB> cur, nxt;
     TileCursor c;
     c.start(a, wave);
     if (c.t < a.ntiles)
-        spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, cur);
+        spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
     while (c.t < a.ntiles) {
+        cur = nxt;
         TileCursor cn = c;
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
         spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
                                                               izm, ntm1, lds);
-        cur = nxt;
         c = cn;
     }
     // with a wavelet: convolved in LDS, y receives the data (see
@@ -855,19 +859,19 @@ __device__ __forceinline__ void spread_models_body(const KArgs &a, const float *
     const int izm = DYN ? a.izmin[s * a.nr + r] : 0;
     const unsigned ntm1 = (unsigned)(a.nt - 1);
 
-    ModelsTile<TT, K> cur, nxt;
+    ModelsTile<TT, K> cur, nxt;  // loads fill nxt only, as in spread_body
     TileCursor c;
     c.start(a, wave);
     if (c.t < a.ntiles)
-        models_load<TT, DYN, K>(a, c, lane, rel, lz, lx, ly, s, r, m, ni, vz, cur);
+        models_load<TT, DYN, K>(a, c, lane, rel, lz, lx, ly, s, r, m, ni, vz, nxt);
     while (c.t < a.ntiles) {
+        cur = nxt;
         TileCursor cn = c;
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             models_load<TT, DYN, K>(a, cn, lane, rel, lz, lx, ly, s, r, m, ni, vz, nxt);
         models_accumulate<TT, DYN, NEG, FIX, K>(cur, c.tzi * a.tz + lz, e, izm, ntm1, a.nt,
                                                 accd, acci);
-        cur = nxt;
         c = cn;
     }
     __syncthreads();
@@ -1229,11 +1233,16 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                     pr[j] = (TD)wv(pr_c[j]);
                     qr[j] = (TD)wv(qr_c[j]);
                 }
-                // next round's receiver side and samples, in flight meanwhile
-                if (r + 1 < a.nr) {
+                // next round's receiver side and samples, in flight meanwhile.
+                // Every round issues the same loads (the last ones reload the
+                // last receiver, unused): with loads under a condition the
+                // compiler cannot count them and waits for this round's
+                // first staged samples before reading the windows.
+                {
+                    const int r1 = min(r + 1, a.nr - 1), r2 = min(r + 2, a.nr - 1);
 #pragma unroll
                     for (int j = 0; j < NP; ++j) {
-                        const size_t ro = (size_t)t[j] * a.nr + r + 1;
+                        const size_t ro = (size_t)t[j] * a.nr + r1;
                         orr_c[j] = offr[ro * kWave + lane];
                         if constexpr (DYN) {
                             pr_c[j] = wpr[ro * kWave + lane];
@@ -1241,12 +1250,10 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                         }
                         br_c[j] = __builtin_amdgcn_readfirstlane(br_n[j]);
                     }
-                    KX_STAGE(r + 1, br_c)
-                    if (r + 2 < a.nr) {
+                    KX_STAGE(r1, br_c)
 #pragma unroll
-                        for (int j = 0; j < NP; ++j)
-                            br_n[j] = a.base_r[(size_t)t[j] * a.nr + r + 2 + vz];
-                    }
+                    for (int j = 0; j < NP; ++j)
+                        br_n[j] = a.base_r[(size_t)t[j] * a.nr + r2 + vz];
                 }
                 // Each lane reads one pair per (source, tile) from the staged
                 // windows; a sample outside its window reads the zero pair
