@@ -1256,38 +1256,44 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                         br_n[j] = a.base_r[(size_t)t[j] * a.nr + r2 + vz];
                 }
                 // Each lane reads one pair per (source, tile) from the staged
-                // windows; a sample outside its window reads the zero pair
-                // and is marked, and the wave adds it afterwards from global
-                // memory, so that the common path has no per-sample branch.
-                unsigned miss = 0;
+                // windows; a sample outside its window reads the zero pair,
+                // and the wave adds it afterwards from global memory, so that
+                // the common path has no per-sample branch.  The common path
+                // keeps only the largest window index of the lane: the rare
+                // wave with a lane outside its windows finds the samples
+                // again with the same index computation.
+                auto window_index = [&](int i, int j, TC &f) -> int {
+                    const int db = bs[i][j] + br[j] - ws[i];  // wave-uniform
+                    const TC T = os[i][j] + orr[j];
+                    int kk;
+                    if constexpr (!NEG && sizeof(TC) == 4) {
+                        f = __builtin_amdgcn_fractf(T);
+                        kk = floor_int(T) + db;
+                    } else {
+                        const TC fl = floor(T);
+                        f = T - fl;
+                        kk = (int)fl + db;
+                        if constexpr (NEG) {
+                            // the reference truncates toward zero: a time in
+                            // (-1, 0) reads sample 0 with a negative weight
+                            if (kk + ws[i] == -1 && f > TC(0)) {
+                                kk += 1;
+                                f -= TC(1);
+                            }
+                        }
+                    }
+                    return kk;
+                };
+                unsigned kmax = 0;
 #pragma unroll
                 for (int i = 0; i < JW; ++i) {
 #pragma unroll
                     for (int j = 0; j < NP; ++j) {
-                        const int db = bs[i][j] + br[j] - ws[i];  // wave-uniform
-                        const TC T = os[i][j] + orr[j];
                         TC f;
-                        int kk;
-                        if constexpr (!NEG && sizeof(TC) == 4) {
-                            f = __builtin_amdgcn_fractf(T);
-                            kk = floor_int(T) + db;
-                        } else {
-                            const TC fl = floor(T);
-                            f = T - fl;
-                            kk = (int)fl + db;
-                            if constexpr (NEG) {
-                                // the reference truncates toward zero: a time
-                                // in (-1, 0) reads sample 0 with a negative
-                                // weight
-                                if (kk + ws[i] == -1 && f > TC(0)) {
-                                    kk += 1;
-                                    f -= TC(1);
-                                }
-                            }
-                        }
-                        const bool inw = (unsigned)kk <= (unsigned)(kWin - 2);
-                        miss |= inw ? 0u : (1u << (i * NP + j));
-                        const int kc = inw ? kk : kWin - 1;
+                        const unsigned kk = (unsigned)window_index(i, j, f);
+                        kmax = max(kmax, kk);
+                        // outside the window: position kWin - 1, the zero pair
+                        const unsigned kc = min(kk, (unsigned)(kWin - 1));
                         TD wgt = TD(1);
                         if constexpr (DYN) {
                             wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
@@ -1304,13 +1310,14 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                         }
                     }
                 }
-                if (__ballot(miss != 0) != 0) {
-                    // the marked samples, from global memory
+                if (__ballot(kmax > (unsigned)(kWin - 2)) != 0) {
+                    // the samples outside the windows, from global memory
 #pragma unroll
                     for (int i = 0; i < JW; ++i) {
 #pragma unroll
                         for (int j = 0; j < NP; ++j) {
-                            if ((miss >> (i * NP + j)) & 1u) {
+                            TC fw;
+                            if ((unsigned)window_index(i, j, fw) > (unsigned)(kWin - 2)) {
                                 const TC T = os[i][j] + orr[j];
                                 const TC fl = floor(T);
                                 TC f = T - fl;
