@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kirchx.h"
 
@@ -677,26 +678,37 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     }
     const unsigned ntm1 = (unsigned)(a.nt - 1);
 
-    // the next tile is loaded while this one is accumulated (a twice
+    // The next tile is loaded while this one is accumulated (a twice
     // unrolled loop with two buffers and no copies measured slower: the
-    // second buffer costs registers and occupancy).  Loads only ever fill
-    // nxt, which is copied to cur at the top of the loop: the wait for a
-    // tile's loads then falls a whole accumulation after they were issued
-    // (loading the first tile into cur makes the compiler wait for the
-    // prefetch before every accumulation).
+    // second buffer costs registers and occupancy).  Two forms of the loop:
+    //   float16 tables: loads only ever fill nxt, copied to cur at the top
+    //     of the loop, so that the wait for a tile's loads falls a whole
+    //     accumulation after they were issued;
+    //   otherwise: the first tile is loaded into cur and nxt copied over it
+    //     at the end, where the compiler waits for the prefetch just issued
+    //     before every accumulation.
+    // On the 3D 128x128x120 workload the first form is 5% faster with
+    // float16 tables and no faster with float32 ones, where its waves drift
+    // apart and the workgroups of one receiver block (one per source) no
+    // longer share its rows in L2: 102 against 57 GB from HBM per launch.
+    constexpr bool kAhead = std::is_same<TT, __half>::value;
     SpreadTile<TD, TT, ORD, RB> cur, nxt;
     TileCursor c;
     c.start(a, wave);
     if (c.t < a.ntiles)
-        spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
+        spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
+                                          kAhead ? nxt : cur);
     while (c.t < a.ntiles) {
-        cur = nxt;
+        if constexpr (kAhead)
+            cur = nxt;
         TileCursor cn = c;
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
         spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
                                                               izm, ntm1, lds);
+        if constexpr (!kAhead)
+            cur = nxt;
         c = cn;
     }
     // with a wavelet: convolved in LDS, y receives the data (see
@@ -859,19 +871,19 @@ __device__ __forceinline__ void spread_models_body(const KArgs &a, const float *
     const int izm = DYN ? a.izmin[s * a.nr + r] : 0;
     const unsigned ntm1 = (unsigned)(a.nt - 1);
 
-    ModelsTile<TT, K> cur, nxt;  // loads fill nxt only, as in spread_body
+    ModelsTile<TT, K> cur, nxt;
     TileCursor c;
     c.start(a, wave);
     if (c.t < a.ntiles)
-        models_load<TT, DYN, K>(a, c, lane, rel, lz, lx, ly, s, r, m, ni, vz, nxt);
+        models_load<TT, DYN, K>(a, c, lane, rel, lz, lx, ly, s, r, m, ni, vz, cur);
     while (c.t < a.ntiles) {
-        cur = nxt;
         TileCursor cn = c;
         cn.advance(a, nw);
         if (cn.t < a.ntiles)
             models_load<TT, DYN, K>(a, cn, lane, rel, lz, lx, ly, s, r, m, ni, vz, nxt);
         models_accumulate<TT, DYN, NEG, FIX, K>(cur, c.tzi * a.tz + lz, e, izm, ntm1, a.nt,
                                                 accd, acci);
+        cur = nxt;
         c = cn;
     }
     __syncthreads();
