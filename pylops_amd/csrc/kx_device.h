@@ -386,6 +386,14 @@ __device__ __forceinline__ int round_half_up(float x)
 }
 __device__ __forceinline__ int round_half_up(double x) { return (int)floor(x + 0.5); }
 
+// min(max(x, 0), hi) in one instruction (hi wave-uniform)
+__device__ __forceinline__ int clamp0(int x, int hi)
+{
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(x), "s"(hi));
+    return r;
+}
+
 // (int)floor(x) in one conversion instruction (|x| < 2^31)
 __device__ __forceinline__ int floor_int(float x)
 {
@@ -1194,7 +1202,7 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
         _Pragma("unroll") for (int j = 0; j < NP; ++j) sum += bs[i][j] + (BR)[j];              \
         wsn[i] = (sum >> Log2<NP>::v) - kWin / 2;                                              \
         const int e = wsn[i] + lane;                                                           \
-        const int ec = min(max(e, 0), a.nt - 1);                                               \
+        const int ec = clamp0(e, a.nt - 1);                                                    \
         _Pragma("unroll") for (int k = 0; k < K; ++k)                                         \
             sv[k][i] = dblk[k * dstride + (unsigned)(soff[i] + (R) * a.nt + ec)];              \
         svok[i] = (unsigned)e < ntm1 && lane < kWin - 1;                                       \
