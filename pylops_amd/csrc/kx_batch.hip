@@ -342,7 +342,9 @@ int per_cu(const void *fn, int block, size_t lds)
     return std::max(n, 1);
 }
 
-size_t spread_lds(int rb, int nt) { return (size_t)rb * nt * 8; }
+// rows trace_stride(nt) samples apart; trace_stride is monotone, so the
+// largest nt of a batch sizes every problem's rows
+size_t spread_lds(int rb, int nt) { return (size_t)rb * trace_stride(nt) * 8; }
 
 // Kernel entry points of a batch, by its element and table types.
 struct Fns {

@@ -506,6 +506,16 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, 
     }
 }
 
+// LDS row stride of the forward's traces, in samples: nt rounded up to 2
+// mod 16.  A 64-bit sample spans two of the 32 banks that serve one 16-lane
+// quarter of an atomic instruction, so the rows of a workgroup's receivers
+// start 2 bank pairs apart; in rotated order the lanes of a quarter on
+// neighbouring receivers, whose samples lie close together, then fall
+// between each other's banks (the 3D spread's own addresses replayed: 5.07
+// atomic lanes per clock and CU at a stride of 1010, 4.18 at 1001, 4.0-4.8
+// at the other residues).  Monotone in nt.
+__host__ __device__ __forceinline__ int trace_stride(int nt) { return nt + ((2 - nt) & 15); }
+
 // Zeroes the ntr samples of a workgroup's LDS traces (0.0 and integer 0
 // share the bit pattern).
 __device__ __forceinline__ void traces_clear(void *lds, int ntr)
@@ -516,27 +526,28 @@ __device__ __forceinline__ void traces_clear(void *lds, int ntr)
     __syncthreads();
 }
 
-// Writes the workgroup's LDS traces (ntr = rows * nt samples) to y: row j
-// to trace first + j * trb; fixed point scaled back by 2^-e, or float64
+// Writes the workgroup's nrl LDS traces (rows lts samples apart) to y: row
+// j to trace first + j * trb; fixed point scaled back by 2^-e, or float64
 // sums rounded to TD.
 template <typename TD, int ACC>
 __device__ __forceinline__ void traces_store(TD *__restrict__ y, size_t first, int trb, int nt,
-                                             int ntr, int e, const void *lds)
+                                             int lts, int nrl, int e, const void *lds)
 {
     __syncthreads();
     TD *yo = y + first * nt;
+    const int ntr = nrl * nt;
     for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
+        const int j = i / nt, n = i - j * nt;
         TD v;
         if constexpr (ACC == kAccFix)
-            v = (TD)ldexp((double)(long long)static_cast<const unsigned long long *>(lds)[i], -e);
+            v = (TD)ldexp(
+                (double)(long long)static_cast<const unsigned long long *>(lds)[j * lts + n], -e);
         else
-            v = (TD) static_cast<const double *>(lds)[i];
-        if (trb == 1) {
+            v = (TD) static_cast<const double *>(lds)[j * lts + n];
+        if (trb == 1)
             yo[i] = v;  // the block's traces are adjacent
-        } else {
-            const int j = i / nt;
-            yo[(size_t)j * trb * nt + (i - j * nt)] = v;
-        }
+        else
+            yo[(size_t)j * trb * nt + n] = v;
     }
     __syncthreads();  // the traces are read out before a next item clears them
 }
@@ -565,14 +576,16 @@ __host__ __device__ __forceinline__ int fused_row(int nt, int nh)
 // sums, the rows 4-byte floats).
 inline bool fused_conv_ok(int rb, int nt, int nh, size_t lds, int block)
 {
-    const int64_t S = fused_row(nt, nh), L = (nh - 1) / 2;
+    const int64_t S = fused_row(nt, nh), L = (nh - 1) / 2, lts = trace_stride(nt);
     if (((int64_t)rb * S + conv_taps_padded(nh)) * 4 > (int64_t)lds)
         return false;
     const int64_t ntr = (int64_t)rb * nt, B = (int64_t)kFusePass * block;
     for (int64_t q = 0; (q + 1) * B < ntr; ++q) {
         const int64_t i = (q + 1) * B - 1;  // last sample of pass q
         const int64_t j = i / nt, n = i - j * nt;
-        if ((j * S + L + n + 1) * 4 > (q + 1) * B * 8)
+        const int64_t i1 = i + 1;  // first sample of pass q + 1, at 8 (j1 lts + n1) bytes
+        const int64_t j1 = i1 / nt, n1 = i1 - j1 * nt;
+        if ((j * S + L + n + 1) * 4 > (j1 * lts + n1) * 8)
             return false;
     }
     return true;
@@ -584,7 +597,7 @@ __device__ __forceinline__ void traces_conv_store(TD *__restrict__ y, size_t fir
                                                   const TD *__restrict__ h, int nh)
 {
     if constexpr (sizeof(TD) == 4) {
-        const int ntr = nrl * nt;
+        const int ntr = nrl * nt, lts = trace_stride(nt);
         const int S = fused_row(nt, nh), L = (nh - 1) / 2, nhp = conv_taps_padded(nh);
         float *win = static_cast<float *>(lds);
         const unsigned long long *acci = static_cast<const unsigned long long *>(lds);
@@ -596,10 +609,11 @@ __device__ __forceinline__ void traces_conv_store(TD *__restrict__ y, size_t fir
             for (int q = 0; q < kFusePass; ++q) {
                 const int i = p0 + q * (int)blockDim.x + (int)threadIdx.x;
                 if (i < ntr) {
+                    const int j = i / nt, li = j * lts + (i - j * nt);
                     if constexpr (ACC == kAccF64)
-                        v[q] = (float)accd[i];
+                        v[q] = (float)accd[li];
                     else
-                        v[q] = (float)ldexp((double)(long long)acci[i], -e);
+                        v[q] = (float)ldexp((double)(long long)acci[li], -e);
                 }
             }
             __syncthreads();  // the pass is read before it is overwritten
@@ -650,7 +664,7 @@ __device__ __forceinline__ void traces_conv_store(TD *__restrict__ y, size_t fir
         }
         __syncthreads();  // the rows are read before a next item clears them
     } else {
-        traces_store<TD, ACC>(y, first, trb, nt, nrl * nt, e, lds);
+        traces_store<TD, ACC>(y, first, trb, nt, trace_stride(nt), nrl, e, lds);
     }
 }
 
@@ -665,9 +679,9 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
 
     // one workgroup = one source s and receivers [r0, r0 + RB)
     const int nrl = min(RB, a.nr - r0);
-    const int ntr = nrl * a.nt;
+    const int lts = trace_stride(a.nt);
 
-    traces_clear(lds, ntr);
+    traces_clear(lds, nrl * lts);
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -680,7 +694,7 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         const int j = step_receiver<RB, ORD>(lane, r);
-        lrow[r] = j < nrl ? j * a.nt : -1;
+        lrow[r] = j < nrl ? j * lts : -1;
         if constexpr (DYN)  // the gate of a pair is indexed like its trace
             izm[r] = a.izmin[(size_t)s * a.trs + (size_t)(r0 + (j < nrl ? j : 0)) * a.trb];
     }
@@ -725,8 +739,8 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
         traces_conv_store<TD, ACC>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, nrl,
                                    e, lds, h, nh);
     else
-        traces_store<TD, ACC>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, ntr, e,
-                              lds);
+        traces_store<TD, ACC>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, lts, nrl,
+                              e, lds);
 }
 
 // One (source, receiver block) item with the launch's accumulation scale:

@@ -49,8 +49,10 @@ constexpr int kRblocks[] = {16, 8, 4, 2, 1};
 constexpr int kLdsPerCu = 160 * 1024;
 constexpr int kMaxWavesPerCu = 32;
 
-// LDS of a workgroup's traces: 8 bytes per sample
-size_t spread_lds_bytes(int rb, int nt) { return (size_t)rb * nt * 8; }
+// LDS of a workgroup's traces: 8 bytes per sample, rows trace_stride(nt)
+// samples apart (the models kernel keeps rows of nt samples)
+size_t spread_lds_bytes(int rb, int nt) { return (size_t)rb * trace_stride(nt) * 8; }
+size_t models_lds_bytes(int k, int nt) { return (size_t)k * nt * 8; }
 
 template <typename TD, typename TT, bool DYN, bool NEG, bool XS, int ORD, int RB>
 int launch_spread_rb(const KArgs &a, const void *m, void *y, const SpreadScale *sc,
@@ -286,7 +288,7 @@ template <typename TT, bool DYN, bool NEG, int K>
 void launch_models_k(const KArgs &a, const float *m, int64_t ni, float *y, int64_t ystride,
                      const SpreadScale *sc, hipStream_t st)
 {
-    const size_t lds = spread_lds_bytes(K, a.nt);
+    const size_t lds = models_lds_bytes(K, a.nt);
     hipLaunchKernelGGL((spread_models_kernel<TT, DYN, NEG, K>), dim3(a.ns * a.nr),
                        dim3(kSpreadBlock), lds, st, a, m, ni, y, ystride, sc);
 }
@@ -312,7 +314,7 @@ int launch_models(const kx_plan *p, const KArgs &a, const float *m, float *y, in
     while (j < nmodel) {
         const int left = nmodel - j;
         auto fits = [&](int k) {
-            return k <= left && spread_lds_bytes(k, a.nt) <= (size_t)kLdsBudget;
+            return k <= left && models_lds_bytes(k, a.nt) <= (size_t)kLdsBudget;
         };
         const float *mj = m + j * ni;
         float *yj = y + j * ystride;
@@ -352,7 +354,7 @@ extern "C" int kx_spread_models(const kx_plan *p, const void *m, void *traces, i
         return kx_failf(KX_ESHAPE, "spread_models: nmodel=%d outside [1, %d]", nmodel, kMaxModels);
     if (!p->aux)
         return kx_failf(KX_EINVAL, "spread_models: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
-    if (spread_lds_bytes(1, p->nt) > (size_t)kLdsBudget)
+    if (models_lds_bytes(1, p->nt) > (size_t)kLdsBudget)
         return kx_failf(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const KArgs a = make_args(p);

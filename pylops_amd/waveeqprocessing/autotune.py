@@ -108,7 +108,8 @@ def _candidates_fwd(dtype, dynamic, nt, sides=(_lib.KX_SIDE_RECEIVERS,), orders=
     if dtype == "float64":
         sides = (_lib.KX_SIDE_RECEIVERS,)  # float64 operators block receivers
     for rb in RBLOCKS:
-        if rb > max_rb or (rb > 1 and rb * nt * 8 > lds):
+        # the forward's LDS rows: nt rounded up to 2 mod 16 samples of 8 bytes
+        if rb > max_rb or (rb > 1 and rb * (nt + (2 - nt) % 16) * 8 > lds):
             continue  # more registers than two workgroups per CU, or LDS
         for so in orders if rb > 1 else (_lib.KX_ORDER_NATURAL,):
             for side in sides:
