@@ -317,17 +317,19 @@ __device__ __forceinline__ int step_base(const int (&br)[BaseLoads<ORD, RB>::n],
         return __builtin_amdgcn_readlane(br[0], r);
     } else if constexpr (RB == 2) {
         // lane l treats receiver (l + r) mod 2: swap lane pairs on odd steps
-        return (r & 1) ? __builtin_amdgcn_update_dpp(0, br[0], 0xB1, 0xf, 0xf, false) : br[0];
+        // (a quad permutation reads no lane outside the wave: mov_dpp needs
+        // no old value)
+        return (r & 1) ? __builtin_amdgcn_mov_dpp(br[0], 0xB1, 0xf, 0xf, true) : br[0];
     } else {
         // RB == 4: lane l treats receiver (l + r) mod 4, held by lane
         // (l & ~3) | ((l + r) & 3): quad_perm [r, r+1, r+2, r+3] mod 4
         switch (r & 3) {
         case 1:
-            return __builtin_amdgcn_update_dpp(0, br[0], 0x39, 0xf, 0xf, false);
+            return __builtin_amdgcn_mov_dpp(br[0], 0x39, 0xf, 0xf, true);
         case 2:
-            return __builtin_amdgcn_update_dpp(0, br[0], 0x4E, 0xf, 0xf, false);
+            return __builtin_amdgcn_mov_dpp(br[0], 0x4E, 0xf, 0xf, true);
         case 3:
-            return __builtin_amdgcn_update_dpp(0, br[0], 0x93, 0xf, 0xf, false);
+            return __builtin_amdgcn_mov_dpp(br[0], 0x93, 0xf, 0xf, true);
         default:
             return br[0];
         }
@@ -700,38 +702,70 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     }
     const unsigned ntm1 = (unsigned)(a.nt - 1);
 
-    // The next tile is loaded while this one is accumulated (a twice
-    // unrolled loop with two buffers and no copies measured slower: the
-    // second buffer costs registers and occupancy).  Two forms of the loop:
-    //   float16 tables: loads only ever fill nxt, copied to cur at the top
-    //     of the loop, so that the wait for a tile's loads falls a whole
-    //     accumulation after they were issued;
-    //   otherwise: the first tile is loaded into cur and nxt copied over it
-    //     at the end, where the compiler waits for the prefetch just issued
-    //     before every accumulation.
-    // On the 3D 128x128x120 workload the first form is 5% faster with
-    // float16 tables and no faster with float32 ones, where its waves drift
-    // apart and the workgroups of one receiver block (one per source) no
-    // longer share its rows in L2: 102 against 57 GB from HBM per launch.
-    constexpr bool kAhead = std::is_same<TT, __half>::value;
-    SpreadTile<TD, TT, ORD, RB> cur, nxt;
+    // The next tile is loaded while this one is accumulated, in one of
+    // three loop forms, chosen by measurement (3D 128x128x120 kinematic,
+    // rotated order; 2D 601x300 dynamic, natural order):
+    //   kFormAhead (float16 tables): loads only ever fill nxt, copied to cur
+    //     at the top of the loop, so that the wait for a tile's loads falls
+    //     a whole accumulation after they were issued (5% faster in 3D);
+    //   kFormPair (float32 kinematic): two tile buffers taken in turn, the
+    //     loop unrolled twice, no copies (2% faster in 3D; 2-6% slower with
+    //     dynamic weights or float16 tables);
+    //   kFormEnd (otherwise): the first tile is loaded into cur and nxt
+    //     copied over it at the end, where the compiler waits for the
+    //     prefetch just issued.  With float32 tables the first form is no
+    //     faster: its waves drift apart and the workgroups of one receiver
+    //     block (one per source) stop sharing its rows in L2 (102 against
+    //     57 GB from HBM per launch in 3D).
+    constexpr int kFormAhead = 0, kFormPair = 1, kFormEnd = 2;
+    constexpr int kForm = std::is_same<TT, __half>::value
+                              ? kFormAhead
+                              : (!DYN && sizeof(TD) == 4 ? kFormPair : kFormEnd);
     TileCursor c;
     c.start(a, wave);
-    if (c.t < a.ntiles)
-        spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
-                                          kAhead ? nxt : cur);
-    while (c.t < a.ntiles) {
-        if constexpr (kAhead)
-            cur = nxt;
-        TileCursor cn = c;
-        cn.advance(a, nw);
-        if (cn.t < a.ntiles)
-            spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, nxt);
-        spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
-                                                              izm, ntm1, lds);
-        if constexpr (!kAhead)
-            cur = nxt;
-        c = cn;
+    if constexpr (kForm == kFormPair) {
+        SpreadTile<TD, TT, ORD, RB> ta, tb;
+        if (c.t < a.ntiles)
+            spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, ta);
+        while (c.t < a.ntiles) {
+            TileCursor cn = c;
+            cn.advance(a, nw);
+            if (cn.t < a.ntiles)
+                spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
+                                                  tb);
+            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(ta, c.tzi * a.tz + lz, e, lrow,
+                                                                  izm, ntm1, lds);
+            c = cn;
+            if (c.t >= a.ntiles)
+                break;
+            cn.advance(a, nw);
+            if (cn.t < a.ntiles)
+                spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
+                                                  ta);
+            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(tb, c.tzi * a.tz + lz, e, lrow,
+                                                                  izm, ntm1, lds);
+            c = cn;
+        }
+    } else {
+        constexpr bool kAhead = kForm == kFormAhead;
+        SpreadTile<TD, TT, ORD, RB> cur, nxt;
+        if (c.t < a.ntiles)
+            spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
+                                              kAhead ? nxt : cur);
+        while (c.t < a.ntiles) {
+            if constexpr (kAhead)
+                cur = nxt;
+            TileCursor cn = c;
+            cn.advance(a, nw);
+            if (cn.t < a.ntiles)
+                spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
+                                                  nxt);
+            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
+                                                                  izm, ntm1, lds);
+            if constexpr (!kAhead)
+                cur = nxt;
+            c = cn;
+        }
     }
     // with a wavelet: convolved in LDS, y receives the data (see
     // traces_conv_store); without: y receives the traces
