@@ -345,9 +345,10 @@ typedef struct kx_batch {
 } kx_batch;
 
 /* Device bytes a batch of these plans needs (descriptors, work lists,
- * counters).  rblock, sorder, gsplit and gtiles of plans[0] apply to the
- * whole batch (0 = automatic; gtiles above 2 counts as 2), the other plans'
- * values are ignored. */
+ * counters), or a negative KX_E* code when the plans do not form a batch
+ * (kx_last_error says why).  rblock, sorder, gsplit and gtiles of plans[0]
+ * apply to the whole batch (0 = automatic; gtiles above 2 counts as 2), the
+ * other plans' values are ignored. */
 int64_t kx_batch_dev_bytes(const kx_plan *plans, int32_t nprob);
 /* Fills *b and writes the descriptors and work lists to dev; returns after
  * the copies are done (not capturable).  The plans' tables and wavelets must

@@ -332,6 +332,7 @@ const void *bgather_fn(int P, int np)
 // wave limits alone.
 int per_cu(const void *fn, int block, size_t lds)
 {
+    allow_lds(fn, lds);
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, lds) != hipSuccess || n <= 0) {
         (void)hipGetLastError();
@@ -442,9 +443,10 @@ int check_batch(const kx_plan *plans, int nprob)
                          "dynamic or neg_trav", p);
         if (!a.h || !a.hstar || a.nh <= 0 || (a.nh % 2) == 0)
             return bfail(KX_EINVAL, "batch: problem %d has no odd-length wavelet", p);
-        if (spread_lds(1, a.nt) > (size_t)kLdsBudget)
-            return bfail(KX_ELDS, "batch: problem %d: nt=%d does not fit one trace in LDS", p,
-                         a.nt);
+        if (spread_lds(1, a.nt) > lds_budget())
+            return bfail(KX_ELDS, "batch: problem %d: nt=%d does not fit one trace in LDS "
+                                  "(the single-problem operator spreads such traces in time "
+                                  "segments)", p, a.nt);
         const size_t clds = (size_t)conv_lds_elems(a.nh) * dsize(a.dtype);
         if (clds > (size_t)kLdsBudget)
             return bfail(KX_ELDS, "batch: problem %d: filter of %d taps does not fit LDS", p,
@@ -474,7 +476,7 @@ Choice choose(const kx_plan *plans, int nprob)
                                          : (q.dynamic ? max_rblock<float, true>()
                                                       : max_rblock<float, false>());
     auto fits = [&](int rb) {
-        return rb <= max_rb && spread_lds(rb, nt_max) <= (size_t)kLdsBudget;
+        return rb <= max_rb && spread_lds(rb, nt_max) <= lds_budget();
     };
     c.rb = 1;
     if (q.rblock > 0) {
@@ -601,6 +603,9 @@ template <typename TD, typename TT, bool DYN, bool NEG, bool ROT, int RB>
 int launch_bspread(const kx_batch *b, const View &v, const void *m, void *y, hipStream_t st)
 {
     if constexpr (RB <= max_rblock<TD, DYN>()) {
+        allow_lds(reinterpret_cast<const void *>(
+                      &batch_spread_kernel<TD, TT, DYN, NEG, ROT && (RB > 1), RB>),
+                  spread_lds(RB, b->nt_max));
         hipLaunchKernelGGL((batch_spread_kernel<TD, TT, DYN, NEG, ROT && (RB > 1), RB>),
                            dim3(b->grid_spread), dim3(kSpreadBlock), spread_lds(RB, b->nt_max),
                            st, v.probs, v.scales, v.sitems, v.ctr + kSpreadQ,
@@ -713,8 +718,9 @@ int launch_bconv(const kx_batch *b, const View &v, const void *x, void *y, int a
 // ---------------------------------------------------------------------------
 extern "C" int64_t kx_batch_dev_bytes(const kx_plan *plans, int32_t nprob)
 {
-    if (check_batch(plans, nprob))
-        return -1;
+    const int rc = check_batch(plans, nprob);
+    if (rc)
+        return rc;
     const Layout L = layout(plans, nprob, choose(plans, nprob));
     return (int64_t)L.total;
 }

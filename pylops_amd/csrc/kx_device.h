@@ -110,6 +110,9 @@ struct KArgs {
     // starting at r0) is trace s * trs + (r0 + j) * trb; (nr, 1) unless the
     // forward blocks sources (the side tables exchanged, then (1, ns))
     int trs, trb;
+    // forward traces held in LDS: samples [t0, t0 + tlen) of every trace,
+    // the whole trace (0, nt) unless a trace does not fit in LDS
+    int t0, tlen;
 };
 
 inline KArgs make_args(const kx_plan *p)
@@ -144,6 +147,8 @@ inline KArgs make_args(const kx_plan *p)
     a.izmin = p->izmin;
     a.trs = p->nr;
     a.trb = 1;
+    a.t0 = 0;
+    a.tlen = p->nt;
     return a;
 }
 
@@ -425,9 +430,12 @@ constexpr int kAccFix = 1;
 // and one fract instruction: fract(T) is T - floor(T) rounded as the
 // subtraction rounds it, except that a result that would round to 1 (T in
 // (-2^-24, 0)) stays just below 1.
-template <typename TD, typename TC, bool NEG, int ACC>
+// SEG: the LDS traces hold samples [t0, t0 + tlen) only; each of the two
+// samples is added where it lies in them.
+template <typename TD, typename TC, bool NEG, int ACC, bool SEG = false>
 __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int lrow,
-                                              unsigned ntm1, void *lds)
+                                              unsigned ntm1, void *lds, int t0 = 0,
+                                              unsigned tlen = 0)
 {
     TC f;
     int k;
@@ -448,20 +456,31 @@ __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int
         }
     }
     if (on && lrow >= 0 && (unsigned)k < ntm1 && w != TD(0)) {
-        const int it = lrow + k;
+        int it = lrow + k;
+        bool in0 = true, in1 = true;
+        if constexpr (SEG) {
+            const int kl = k - t0;
+            it = lrow + kl;
+            in0 = (unsigned)kl < tlen;
+            in1 = (unsigned)(kl + 1) < tlen;
+        }
         const TD w1 = w * (TD)f;
         if constexpr (ACC == kAccFix) {
             // w (1 - f) in one rounding; fixed point by floor(x + 1/2) in
             // one conversion
             unsigned long long *acci = static_cast<unsigned long long *>(lds);
             const TD w0 = fma(-w, (TD)f, w);
-            atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
-            atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
+            if (in0)
+                atomicAdd(&acci[it], (unsigned long long)(long long)round_half_up(w0));
+            if (in1)
+                atomicAdd(&acci[it + 1], (unsigned long long)(long long)round_half_up(w1));
         } else {
             double *accd = static_cast<double *>(lds);
             const TD w0 = w * (TD)(TC(1) - f);
-            atomicAdd(&accd[it], (double)w0);
-            atomicAdd(&accd[it + 1], (double)w1);
+            if (in0)
+                atomicAdd(&accd[it], (double)w0);
+            if (in1)
+                atomicAdd(&accd[it + 1], (double)w1);
         }
     }
 }
@@ -472,11 +491,12 @@ __device__ __forceinline__ void spread_sample(TC T, int base, TD w, bool on, int
 // members are sources) the same products are formed per step, so that the
 // weights round as they do with blocks of receivers.
 template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int ACC, int RB,
-          bool XS = false>
+          bool XS = false, bool SEG = false>
 __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, RB> &cur, int iz,
                                                   int e,
                                                   const int (&lrow)[RB], const int (&izm)[RB],
-                                                  unsigned ntm1, void *lds)
+                                                  unsigned ntm1, void *lds, int t0 = 0,
+                                                  unsigned tlen = 0)
 {
     using TC = typename TimeType<TD>::type;
     if (__ballot(cur.mv != TD(0)) == 0)
@@ -503,8 +523,8 @@ __device__ __forceinline__ void spread_accumulate(const SpreadTile<TD, TT, ORD, 
                 w = fma(mqs, (TD)wv(cur.qr[r]), mps * (TD)wv(cur.pr[r]));
             on = iz >= izm[r];
         }
-        spread_sample<TD, TC, NEG, ACC>(T, cur.bs + step_base<ORD, RB>(cur.br, r), w, on,
-                                        lrow[r], ntm1, lds);
+        spread_sample<TD, TC, NEG, ACC, SEG>(T, cur.bs + step_base<ORD, RB>(cur.br, r), w, on,
+                                             lrow[r], ntm1, lds, t0, tlen);
     }
 }
 
@@ -528,28 +548,28 @@ __device__ __forceinline__ void traces_clear(void *lds, int ntr)
     __syncthreads();
 }
 
-// Writes the workgroup's nrl LDS traces (rows lts samples apart) to y: row
-// j to trace first + j * trb; fixed point scaled back by 2^-e, or float64
-// sums rounded to TD.
+// Writes the workgroup's nrl LDS traces (rows lts samples apart, samples
+// [t0, t0 + tlen) of traces of nt samples) to y: row j to trace first +
+// j * trb; fixed point scaled back by 2^-e, or float64 sums rounded to TD.
 template <typename TD, int ACC>
 __device__ __forceinline__ void traces_store(TD *__restrict__ y, size_t first, int trb, int nt,
-                                             int lts, int nrl, int e, const void *lds)
+                                             int lts, int nrl, int e, const void *lds,
+                                             int t0 = 0, int tlen = -1)
 {
     __syncthreads();
-    TD *yo = y + first * nt;
-    const int ntr = nrl * nt;
+    if (tlen < 0)
+        tlen = nt;
+    TD *yo = y + first * nt + t0;
+    const int ntr = nrl * tlen;
     for (int i = threadIdx.x; i < ntr; i += blockDim.x) {
-        const int j = i / nt, n = i - j * nt;
+        const int j = i / tlen, n = i - j * tlen;
         TD v;
         if constexpr (ACC == kAccFix)
             v = (TD)ldexp(
                 (double)(long long)static_cast<const unsigned long long *>(lds)[j * lts + n], -e);
         else
             v = (TD) static_cast<const double *>(lds)[j * lts + n];
-        if (trb == 1)
-            yo[i] = v;  // the block's traces are adjacent
-        else
-            yo[(size_t)j * trb * nt + n] = v;
+        yo[(size_t)j * trb * nt + n] = v;
     }
     __syncthreads();  // the traces are read out before a next item clears them
 }
@@ -670,8 +690,10 @@ __device__ __forceinline__ void traces_conv_store(TD *__restrict__ y, size_t fir
     }
 }
 
+// SEG: the LDS traces hold samples [a.t0, a.t0 + a.tlen) of each trace (a
+// trace longer than LDS, spread one segment per workgroup; no wavelet).
 template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int ACC, int RB,
-          bool XS = false>
+          bool XS = false, bool SEG = false>
 __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, int e, int s, int r0,
                                             const TD *__restrict__ h = nullptr, int nh = 0)
@@ -681,7 +703,7 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
 
     // one workgroup = one source s and receivers [r0, r0 + RB)
     const int nrl = min(RB, a.nr - r0);
-    const int lts = trace_stride(a.nt);
+    const int lts = trace_stride(SEG ? a.tlen : a.nt);
 
     traces_clear(lds, nrl * lts);
 
@@ -733,8 +755,8 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
             if (cn.t < a.ntiles)
                 spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
                                                   tb);
-            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(ta, c.tzi * a.tz + lz, e, lrow,
-                                                                  izm, ntm1, lds);
+            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS, SEG>(
+                ta, c.tzi * a.tz + lz, e, lrow, izm, ntm1, lds, a.t0, (unsigned)a.tlen);
             c = cn;
             if (c.t >= a.ntiles)
                 break;
@@ -742,8 +764,8 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
             if (cn.t < a.ntiles)
                 spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
                                                   ta);
-            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(tb, c.tzi * a.tz + lz, e, lrow,
-                                                                  izm, ntm1, lds);
+            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS, SEG>(
+                tb, c.tzi * a.tz + lz, e, lrow, izm, ntm1, lds, a.t0, (unsigned)a.tlen);
             c = cn;
         }
     } else {
@@ -760,8 +782,8 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
             if (cn.t < a.ntiles)
                 spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
                                                   nxt);
-            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS>(cur, c.tzi * a.tz + lz, e, lrow,
-                                                                  izm, ntm1, lds);
+            spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS, SEG>(
+                cur, c.tzi * a.tz + lz, e, lrow, izm, ntm1, lds, a.t0, (unsigned)a.tlen);
             if constexpr (!kAhead)
                 cur = nxt;
             c = cn;
@@ -769,18 +791,19 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     }
     // with a wavelet: convolved in LDS, y receives the data (see
     // traces_conv_store); without: y receives the traces
-    if (h != nullptr)
+    if (!SEG && h != nullptr)
         traces_conv_store<TD, ACC>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, nrl,
                                    e, lds, h, nh);
     else
         traces_store<TD, ACC>(y, (size_t)s * a.trs + (size_t)r0 * a.trb, a.trb, a.nt, lts, nrl,
-                              e, lds);
+                              e, lds, SEG ? a.t0 : 0, SEG ? a.tlen : a.nt);
 }
 
 // One (source, receiver block) item with the launch's accumulation scale:
 // float32 data in fixed point, or in float64 when the launch's scale says
 // so (a non-finite model); float64 data always in float64.
-template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int RB, bool XS = false>
+template <typename TD, typename TT, bool DYN, bool NEG, int ORD, int RB, bool XS = false,
+          bool SEG = false>
 __device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict__ m,
                                             TD *__restrict__ y, const SpreadScale &sc, int s,
                                             int r0, const TD *__restrict__ h = nullptr,
@@ -788,11 +811,12 @@ __device__ __forceinline__ void spread_item(const KArgs &a, const TD *__restrict
 {
     if constexpr (sizeof(TD) == 4) {
         if (sc.fixed)
-            spread_body<TD, TT, DYN, NEG, ORD, kAccFix, RB, XS>(a, m, y, sc.e, s, r0, h, nh);
+            spread_body<TD, TT, DYN, NEG, ORD, kAccFix, RB, XS, SEG>(a, m, y, sc.e, s, r0, h,
+                                                                     nh);
         else
-            spread_body<TD, TT, DYN, NEG, ORD, kAccF64, RB, XS>(a, m, y, 0, s, r0, h, nh);
+            spread_body<TD, TT, DYN, NEG, ORD, kAccF64, RB, XS, SEG>(a, m, y, 0, s, r0, h, nh);
     } else {
-        spread_body<TD, TT, DYN, NEG, ORD, kAccF64, RB, XS>(a, m, y, 0, s, r0);
+        spread_body<TD, TT, DYN, NEG, ORD, kAccF64, RB, XS, SEG>(a, m, y, 0, s, r0);
     }
 }
 

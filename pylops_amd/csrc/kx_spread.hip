@@ -26,6 +26,26 @@ __global__ __launch_bounds__(kSpreadBlock) void spread_kernel(KArgs a, const TD 
     spread_item<TD, TT, DYN, NEG, ORD, RB, XS>(a, m, y, scale, L % a.ns, (L / a.ns) * RB, h, nh);
 }
 
+// Traces longer than a workgroup's LDS: workgroup L (after the XCD
+// renumbering) = segment L mod nseg of the trace of source (L / nseg) mod
+// ns and receiver L / (nseg ns), samples [seg tseg, seg tseg + tseg); the
+// segments of one trace are neighbours and share its table rows in L2.
+template <typename TD, typename TT, bool DYN, bool NEG>
+__global__ __launch_bounds__(kSpreadBlock) void spread_seg_kernel(KArgs a,
+                                                                  const TD *__restrict__ m,
+                                                                  TD *__restrict__ y,
+                                                                  const SpreadScale *__restrict__ sc,
+                                                                  int tseg, int nseg)
+{
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int seg = L % nseg, sr = L / nseg;
+    a.t0 = seg * tseg;
+    a.tlen = min(tseg, a.nt - a.t0);
+    const SpreadScale scale = *sc;
+    spread_item<TD, TT, DYN, NEG, kOrdNatural, 1, false, true>(a, m, y, scale, sr % a.ns,
+                                                               sr / a.ns);
+}
+
 // K models of a float32 operator; workgroup L (after the XCD renumbering) =
 // source L mod ns, receiver L / ns.  The launch accumulates in fixed point
 // when every model's scale allows it, in float64 otherwise.
@@ -175,6 +195,37 @@ int launch_spread_t(const KArgs &a, int order, int requested, int max_rb, const 
 // h != NULL: the spread and the wavelet convolution in one launch (y
 // receives the data), or KX_ENOFUSE when this plan's launch cannot
 // convolve in LDS (float64 data, too little LDS).
+// Segments of traces that do not fit a workgroup's LDS: the longest
+// multiple of 64 samples that does, or 0 if not even 64 fit.
+int trace_segment(int nt)
+{
+    int t = (int)std::min<size_t>((size_t)nt, lds_budget() / 8) & ~63;
+    while (t > 0 && spread_lds_bytes(1, t) > lds_budget())
+        t -= 64;
+    return t;
+}
+
+template <typename TD, typename TT, bool DYN, bool NEG>
+int launch_spread_seg(const KArgs &a, const void *m, void *y, const SpreadScale *sc,
+                      hipStream_t st)
+{
+    const int tseg = trace_segment(a.nt);
+    if (tseg <= 0)
+        return kx_failf(KX_ELDS, "nt=%d: not even 64 samples of a trace fit in LDS", a.nt);
+    const int nseg = (a.nt + tseg - 1) / tseg;
+    const int64_t nblk = (int64_t)a.ns * a.nr * nseg;
+    if (nblk > INT32_MAX)
+        return kx_failf(KX_ESHAPE, "spread: %lld trace segments exceed the grid",
+                        (long long)nblk);
+    const size_t lds = spread_lds_bytes(1, tseg);
+    allow_lds(reinterpret_cast<const void *>(&spread_seg_kernel<TD, TT, DYN, NEG>), lds);
+    hipLaunchKernelGGL((spread_seg_kernel<TD, TT, DYN, NEG>), dim3((unsigned)nblk),
+                       dim3(kSpreadBlock), lds, st, a, reinterpret_cast<const TD *>(m),
+                       reinterpret_cast<TD *>(y), sc, tseg, nseg);
+    KX_LAUNCHED("spread segments");
+    return KX_OK;
+}
+
 template <typename TD, typename TT>
 int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream_t st,
                     const void *h = nullptr)
@@ -182,8 +233,11 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
     const int nh = h != nullptr ? p->nh : 0;
     if (h != nullptr && sizeof(TD) != 4)
         return KX_ENOFUSE;
-    if (spread_lds_bytes(1, p->nt) > lds_budget())
-        return kx_failf(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
+    // a trace longer than LDS: spread in time segments (the wavelet then
+    // convolves the traces in HBM)
+    const bool seg = spread_lds_bytes(1, p->nt) > lds_budget();
+    if (seg && h != nullptr)
+        return KX_ENOFUSE;
     if (sizeof(TD) == 4 && !p->aux)
         return kx_failf(KX_EINVAL, "spread: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
     // registers per lane grow with the block: keep two workgroups per CU
@@ -216,6 +270,12 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
     // whose rows then serve many workgroups through L2 (13% faster on a
     // 61 x 301 geometry, either way round); 3D and dynamic plans block
     // receivers (measured no faster otherwise).
+    if (seg)
+        return p->dynamic ? (p->neg_trav ? launch_spread_seg<TD, TT, true, true>(a, m, y, sc, st)
+                                         : launch_spread_seg<TD, TT, true, false>(a, m, y, sc, st))
+                          : (p->neg_trav ? launch_spread_seg<TD, TT, false, true>(a, m, y, sc, st)
+                                         : launch_spread_seg<TD, TT, false, false>(a, m, y, sc,
+                                                                                   st));
     if (xs)
         exchange_sides(a);
     // automatic lane order: rotated when a step loads one receiver table,
@@ -354,8 +414,8 @@ extern "C" int kx_spread_models(const kx_plan *p, const void *m, void *traces, i
         return kx_failf(KX_ESHAPE, "spread_models: nmodel=%d outside [1, %d]", nmodel, kMaxModels);
     if (!p->aux)
         return kx_failf(KX_EINVAL, "spread_models: plan.aux (KX_AUX_BYTES of device scratch) is NULL");
-    if (models_lds_bytes(1, p->nt) > (size_t)kLdsBudget)
-        return kx_failf(KX_ELDS, "nt=%d does not fit one trace in LDS", p->nt);
+    // (traces of a few models per workgroup that do not fit LDS fall back to
+    // single spreads, in time segments if need be)
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const KArgs a = make_args(p);
     const float *mf = reinterpret_cast<const float *>(m);

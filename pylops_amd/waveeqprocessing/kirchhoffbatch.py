@@ -68,7 +68,7 @@ class KirchhoffBatch(LinearOperator):
         L = _lib.load()
         nbytes = L.kx_batch_dev_bytes(plans, n)
         if nbytes < 0:
-            _lib.check(_lib.KX_EINVAL, "kx_batch_dev_bytes")
+            _lib.check(int(nbytes), "kx_batch_dev_bytes")
         # the device tables of every operator stay alive with the batch, even
         # if an operator rebuilds its own
         self._keep = [op._dev_tables for op in ops]

@@ -229,6 +229,39 @@ def test_cgls_batch_graph_equals_eager():
     assert np.all(s2.iiter == 6)
 
 
+def test_batch_long_traces_beyond_64k_lds():
+    """Traces of 12000 samples (96 KB of LDS each, more than the 64 KB every
+    CDNA device offers a workgroup, within gfx950's 160 KB) batch like short
+    ones: the batched forward is bitwise each operator's and the adjoint
+    agrees; traces that fit no workgroup are refused with the reason."""
+    import pylops_amd
+    from pylops_amd.waveeqprocessing import KirchhoffBatch
+    kws = []
+    for nt, nx in ((12000, 10), (11000, 14)):
+        t = np.arange(nt) * 0.0005
+        x = np.arange(nx) * 10.0
+        z = 500.0 + np.arange(12) * 10.0
+        wav, _, wc = O.ricker(t[:41], f0=40)
+        kws.append(dict(z=z, x=x, t=t, srcs=np.vstack((np.array([0.0, x[-1]]), np.zeros(2))),
+                        recs=np.vstack((np.linspace(0.0, x[-1], 3), np.zeros(3))), vel=300.0,
+                        wav=wav.astype(np.float32), wavcenter=wc, mode="analytic",
+                        dtype="float32"))
+    ops = [pylops_amd.Kirchhoff(**kw) for kw in kws]
+    B = KirchhoffBatch(ops)
+    rng = np.random.default_rng(5)
+    m = rng.standard_normal(B.shape[1]).astype(np.float32)
+    d = rng.standard_normal(B.shape[0]).astype(np.float32)
+    fb, ab = B @ m, B.H @ d
+    mo, do = B.model_offsets, B.data_offsets
+    assert np.abs(fb).max() > 0
+    for p, op in enumerate(ops):
+        np.testing.assert_array_equal(fb[do[p]:do[p + 1]], op @ m[mo[p]:mo[p + 1]])
+        assert G.relerr(ab[mo[p]:mo[p + 1]], op.H @ d[do[p]:do[p + 1]]) < 1e-6
+    long_kw = dict(kws[0], t=np.arange(24000) * 0.0005)
+    with pytest.raises(NotImplementedError, match="does not fit one trace in LDS"):
+        KirchhoffBatch([pylops_amd.Kirchhoff(**long_kw)])
+
+
 def test_batch_rejects_mixed_table_formats():
     import pylops_amd
     from pylops_amd.waveeqprocessing import KirchhoffBatch

@@ -682,3 +682,47 @@ def test_forward_convolves_in_the_spread_launch(case):
                 np.testing.assert_array_equal(fused[ok], two[ok])
             else:
                 np.testing.assert_array_equal(fused, two, err_msg=f"order={order} rb={rb}")
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_long_traces_in_time_segments(dynamic):
+    """Traces longer than a workgroup's LDS (24000 samples; on gfx950 one
+    workgroup holds 20416) are spread in time segments, one per workgroup,
+    the pair of samples of a contribution split where it straddles a
+    segment boundary; the forward (segments, then the wavelet in HBM), the
+    adjoint, a float64 operator and a batch of models (matmat, which then
+    spreads model by model) match the oracle.  A 1 m image 2 km deep puts
+    its samples on both sides of sample 20416 (2.04 s at 0.1 ms)."""
+    from oracle import oracle as Ox
+    nt, dt = 24000, 0.0001
+    t = np.arange(nt) * dt
+    x = np.arange(8) * 1.0
+    z = 1990.0 + np.arange(8) * 1.0
+    srcs = np.vstack((np.array([-600.0, 600.0]), np.zeros(2)))
+    recs = np.vstack((np.array([-300.0, 0.0, 300.0]), np.zeros(3)))
+    wav, _, wc = Ox.ricker(t[:41], f0=500)
+    kw = dict(z=z, x=x, t=t, srcs=srcs, recs=recs, vel=2000.0, wav=wav.astype(np.float32),
+              wavcenter=wc, mode="analytic", dynamic=dynamic, dtype="float32")
+    if dynamic:
+        # the offset aperture is |six - rix| / (iz + 1) in grid indices (K:836-880):
+        # sources 600 grid points away over 8 depths need a wide one
+        kw.update(angleaperture=89, aperture=1e6)
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    rng = np.random.default_rng(41)
+    m = rng.standard_normal(Op.shape[1]).astype(np.float32)
+    d = np.asarray(Op @ m)
+    want = orc.matvec(m)
+    k = np.nonzero(np.abs(want.reshape(-1, nt)).max(axis=0) > 0)[0]
+    assert k.min() < 20416 < k.max(), (k.min(), k.max())  # both segments hold samples
+    assert G.relerr(d, want) < 1e-5
+    assert G.relerr(np.asarray(Op.H @ want.astype(np.float32)), orc.rmatvec(want)) < 1e-5
+    M = rng.standard_normal((Op.shape[1], 2)).astype(np.float32)
+    D = np.asarray(Op.matmat(M))
+    for j in range(2):
+        assert G.relerr(D[:, j], orc.matvec(M[:, j])) < 1e-5
+    kw64 = dict(kw, dtype="float64", wav=wav)
+    Op64 = _K(**kw64)
+    orc64 = O.KirchhoffOracle(**kw64)
+    assert G.relerr(np.asarray(Op64 @ m.astype(np.float64)),
+                    orc64.matvec(m.astype(np.float64))) < 1e-10
