@@ -1144,6 +1144,10 @@ __device__ __forceinline__ double lane_next(double v)
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+template <int N> struct IntC {
+    static constexpr int value = N;
+};
+
 template <int NP> struct Log2 {
     static constexpr int v = NP >= 4 ? 2 : (NP >= 2 ? 1 : 0);
 };
@@ -1223,52 +1227,56 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
 #pragma unroll
         for (int j = 0; j < NP; ++j)
             acc[k][j] = TD(0);
-    if (active) {
-        for (int s0 = 0; s0 < a.ns; s0 += SB) {
-            // this wave's sources s0 + part + P i: offsets per point, bases
-            // per tile (wave-uniform, held in scalar registers)
-            TC os[JW][NP];
-            int bs[JW][NP];
-            TD ps[JW][NP], qs[JW][NP];
-            // trace (source, receiver 0) as an element offset from the
-            // block's first trace (fits 31 bits: checked at launch), held in
-            // vector registers: JW 64-bit pointers would exhaust the scalar
-            // ones
-            const TD *dblk = d + (size_t)s0 * a.nr * a.nt;
-            int soff[JW], izoff[JW];
+    // One block of SB sources, of which this wave's slots i < JB are
+    // summed: JB = JW, or JW / 2 in a last block where at most half of
+    // the wave's slots hold a source (an empty slot adds zero pairs at the
+    // full arithmetic cost of a source).
+    auto source_block = [&](auto jw, const int s0) __attribute__((always_inline)) {
+        constexpr int JB = decltype(jw)::value;
+        // this wave's sources s0 + part + P i: offsets per point, bases
+        // per tile (wave-uniform, held in scalar registers)
+        TC os[JB][NP];
+        int bs[JB][NP];
+        TD ps[JB][NP], qs[JB][NP];
+        // trace (source, receiver 0) as an element offset from the
+        // block's first trace (fits 31 bits: checked at launch), held in
+        // vector registers: JB 64-bit pointers would exhaust the scalar
+        // ones
+        const TD *dblk = d + (size_t)s0 * a.nr * a.nt;
+        int soff[JB], izoff[JB];
 #pragma unroll
-            for (int i = 0; i < JW; ++i) {
-                const int js = s0 + part + P * i;
+        for (int i = 0; i < JB; ++i) {
+            const int js = s0 + part + P * i;
 #pragma unroll
-                for (int j = 0; j < NP; ++j) {
-                    if (js < a.ns) {
-                        const size_t so = (size_t)t[j] * a.ns + js;
-                        os[i][j] = (TC)ld_off(offs + so * kWave + lane);
-                        bs[i][j] = __builtin_amdgcn_readfirstlane(a.base_s[so + vz]);
-                        if constexpr (DYN) {
-                            ps[i][j] = wps[so * kWave + lane];
-                            qs[i][j] = wqs[so * kWave + lane];
-                        }
-                    } else {  // past the last source: every sample before the trace
-                        os[i][j] = TC(0);
-                        bs[i][j] = kFar;
-                        ps[i][j] = qs[i][j] = TD(0);
+            for (int j = 0; j < NP; ++j) {
+                if (js < a.ns) {
+                    const size_t so = (size_t)t[j] * a.ns + js;
+                    os[i][j] = (TC)ld_off(offs + so * kWave + lane);
+                    bs[i][j] = __builtin_amdgcn_readfirstlane(a.base_s[so + vz]);
+                    if constexpr (DYN) {
+                        ps[i][j] = wps[so * kWave + lane];
+                        qs[i][j] = wqs[so * kWave + lane];
                     }
+                } else {  // past the last source: every sample before the trace
+                    os[i][j] = TC(0);
+                    bs[i][j] = kFar;
+                    ps[i][j] = qs[i][j] = TD(0);
                 }
-                soff[i] = (min(js, a.ns - 1) - s0) * a.nr * a.nt + vz;
-                izoff[i] = min(js, a.ns - 1) * a.nr + vz;
             }
+            soff[i] = (min(js, a.ns - 1) - s0) * a.nr * a.nt + vz;
+            izoff[i] = min(js, a.ns - 1) * a.nr + vz;
+        }
 
-            // the window of source i in round r starts at ws = mean over the
-            // group's tiles of bs + br(r), minus kWin / 2; lane l stages
-            // sample ws + l (a clamped load; svok: the pair of position l lies
-            // in [0, nt - 2] and has a neighbour), the aperture gate of the
-            // round is loaded with it
-            TD sv[K][JW];
-            bool svok[JW];
-            int wsn[JW], izn[JW] = {};
+        // the window of source i in round r starts at ws = mean over the
+        // group's tiles of bs + br(r), minus kWin / 2; lane l stages
+        // sample ws + l (a clamped load; svok: the pair of position l lies
+        // in [0, nt - 2] and has a neighbour), the aperture gate of the
+        // round is loaded with it
+        TD sv[K][JB];
+        bool svok[JB];
+        int wsn[JB], izn[JB] = {};
 #define KX_STAGE(R, BR)                                                                        \
-    _Pragma("unroll") for (int i = 0; i < JW; ++i)                                              \
+    _Pragma("unroll") for (int i = 0; i < JB; ++i)                                              \
     {                                                                                          \
         int sum = 0;                                                                           \
         _Pragma("unroll") for (int j = 0; j < NP; ++j) sum += bs[i][j] + (BR)[j];              \
@@ -1281,168 +1289,177 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
         if constexpr (DYN)                                                                     \
             izn[i] = a.izmin[(unsigned)(izoff[i] + (R))];                                      \
     }
-            // receiver side of the round ahead; receiver bases two rounds
-            // ahead (the window positions of the round ahead depend on them)
-            TT orr_c[NP];
-            int br_c[NP], br_n[NP];
-            RW pr_c[NP], qr_c[NP];
+        // receiver side of the round ahead; receiver bases two rounds
+        // ahead (the window positions of the round ahead depend on them)
+        TT orr_c[NP];
+        int br_c[NP], br_n[NP];
+        RW pr_c[NP], qr_c[NP];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const size_t ro = (size_t)t[j] * a.nr;
+            orr_c[j] = offr[ro * kWave + lane];
+            br_c[j] = __builtin_amdgcn_readfirstlane(a.base_r[ro + vz]);
+            br_n[j] = a.nr > 1 ? a.base_r[ro + 1 + vz] : 0;
+            pr_c[j] = qr_c[j] = RW(0.0f);
+            if constexpr (DYN) {
+                pr_c[j] = wpr[ro * kWave + lane];
+                qr_c[j] = wqr[ro * kWave + lane];
+            }
+        }
+        KX_STAGE(0, br_c)
+        for (int r = 0; r < a.nr; ++r) {
+            // this round's pairs into the wave's LDS region
+            int ws[JB], izr[JB];
+#pragma unroll
+            for (int i = 0; i < JB; ++i) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const TD nx = lane_next(sv[k][i]);
+                    TP q;
+                    q.v[0] = svok[i] ? sv[k][i] : TD(0);
+                    q.v[1] = svok[i] ? nx - sv[k][i] : TD(0);
+                    w[k][i][lane] = q;
+                }
+                ws[i] = wsn[i];
+                izr[i] = izn[i];
+            }
+            TC orr[NP];
+            int br[NP];
+            TD pr[NP], qr[NP];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
-                const size_t ro = (size_t)t[j] * a.nr;
-                orr_c[j] = offr[ro * kWave + lane];
-                br_c[j] = __builtin_amdgcn_readfirstlane(a.base_r[ro + vz]);
-                br_n[j] = a.nr > 1 ? a.base_r[ro + 1 + vz] : 0;
-                pr_c[j] = qr_c[j] = RW(0.0f);
-                if constexpr (DYN) {
-                    pr_c[j] = wpr[ro * kWave + lane];
-                    qr_c[j] = wqr[ro * kWave + lane];
-                }
+                orr[j] = (TC)ld_off(&orr_c[j]);
+                br[j] = br_c[j];
+                pr[j] = (TD)wv(pr_c[j]);
+                qr[j] = (TD)wv(qr_c[j]);
             }
-            KX_STAGE(0, br_c)
-            for (int r = 0; r < a.nr; ++r) {
-                // this round's pairs into the wave's LDS region
-                int ws[JW], izr[JW];
-#pragma unroll
-                for (int i = 0; i < JW; ++i) {
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const TD nx = lane_next(sv[k][i]);
-                        TP q;
-                        q.v[0] = svok[i] ? sv[k][i] : TD(0);
-                        q.v[1] = svok[i] ? nx - sv[k][i] : TD(0);
-                        w[k][i][lane] = q;
-                    }
-                    ws[i] = wsn[i];
-                    izr[i] = izn[i];
-                }
-                TC orr[NP];
-                int br[NP];
-                TD pr[NP], qr[NP];
+            // next round's receiver side and samples, in flight meanwhile.
+            // Every round issues the same loads (the last ones reload the
+            // last receiver, unused): with loads under a condition the
+            // compiler cannot count them and waits for this round's
+            // first staged samples before reading the windows.
+            {
+                const int r1 = min(r + 1, a.nr - 1), r2 = min(r + 2, a.nr - 1);
 #pragma unroll
                 for (int j = 0; j < NP; ++j) {
-                    orr[j] = (TC)ld_off(&orr_c[j]);
-                    br[j] = br_c[j];
-                    pr[j] = (TD)wv(pr_c[j]);
-                    qr[j] = (TD)wv(qr_c[j]);
+                    const size_t ro = (size_t)t[j] * a.nr + r1;
+                    orr_c[j] = offr[ro * kWave + lane];
+                    if constexpr (DYN) {
+                        pr_c[j] = wpr[ro * kWave + lane];
+                        qr_c[j] = wqr[ro * kWave + lane];
+                    }
+                    br_c[j] = __builtin_amdgcn_readfirstlane(br_n[j]);
                 }
-                // next round's receiver side and samples, in flight meanwhile.
-                // Every round issues the same loads (the last ones reload the
-                // last receiver, unused): with loads under a condition the
-                // compiler cannot count them and waits for this round's
-                // first staged samples before reading the windows.
-                {
-                    const int r1 = min(r + 1, a.nr - 1), r2 = min(r + 2, a.nr - 1);
+                KX_STAGE(r1, br_c)
+#pragma unroll
+                for (int j = 0; j < NP; ++j)
+                    br_n[j] = a.base_r[(size_t)t[j] * a.nr + r2 + vz];
+            }
+            // Each lane reads one pair per (source, tile) from the staged
+            // windows; a sample outside its window reads the zero pair,
+            // and the wave adds it afterwards from global memory, so that
+            // the common path has no per-sample branch.  The common path
+            // keeps only the largest window index of the lane: the rare
+            // wave with a lane outside its windows finds the samples
+            // again with the same index computation.
+            auto window_index = [&](int i, int j, TC &f) -> int {
+                const int db = bs[i][j] + br[j] - ws[i];  // wave-uniform
+                const TC T = os[i][j] + orr[j];
+                int kk;
+                if constexpr (!NEG && sizeof(TC) == 4) {
+                    f = __builtin_amdgcn_fractf(T);
+                    kk = floor_int(T) + db;
+                } else {
+                    const TC fl = floor(T);
+                    f = T - fl;
+                    kk = (int)fl + db;
+                    if constexpr (NEG) {
+                        // the reference truncates toward zero: a time in
+                        // (-1, 0) reads sample 0 with a negative weight
+                        if (kk + ws[i] == -1 && f > TC(0)) {
+                            kk += 1;
+                            f -= TC(1);
+                        }
+                    }
+                }
+                return kk;
+            };
+            unsigned kmax = 0;
+#pragma unroll
+            for (int i = 0; i < JB; ++i) {
+#pragma unroll
+                for (int j = 0; j < NP; ++j) {
+                    TC f;
+                    const unsigned kk = (unsigned)window_index(i, j, f);
+                    kmax = max(kmax, kk);
+                    // outside the window: position kWin - 1, the zero pair
+                    const unsigned kc = min(kk, (unsigned)(kWin - 1));
+                    TD wgt = TD(1);
+                    if constexpr (DYN) {
+                        wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
+                        if (pt[j].iz < izr[i])
+                            wgt = TD(0);
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const TP q = w[k][i][kc];
+                        TD v = fma((TD)f, q.v[1], q.v[0]);
+                        if constexpr (DYN)
+                            v *= wgt;
+                        acc[k][j] += v;
+                    }
+                }
+            }
+            if (__ballot(kmax > (unsigned)(kWin - 2)) != 0) {
+                // the samples outside the windows, from global memory
+#pragma unroll
+                for (int i = 0; i < JB; ++i) {
 #pragma unroll
                     for (int j = 0; j < NP; ++j) {
-                        const size_t ro = (size_t)t[j] * a.nr + r1;
-                        orr_c[j] = offr[ro * kWave + lane];
-                        if constexpr (DYN) {
-                            pr_c[j] = wpr[ro * kWave + lane];
-                            qr_c[j] = wqr[ro * kWave + lane];
-                        }
-                        br_c[j] = __builtin_amdgcn_readfirstlane(br_n[j]);
-                    }
-                    KX_STAGE(r1, br_c)
-#pragma unroll
-                    for (int j = 0; j < NP; ++j)
-                        br_n[j] = a.base_r[(size_t)t[j] * a.nr + r2 + vz];
-                }
-                // Each lane reads one pair per (source, tile) from the staged
-                // windows; a sample outside its window reads the zero pair,
-                // and the wave adds it afterwards from global memory, so that
-                // the common path has no per-sample branch.  The common path
-                // keeps only the largest window index of the lane: the rare
-                // wave with a lane outside its windows finds the samples
-                // again with the same index computation.
-                auto window_index = [&](int i, int j, TC &f) -> int {
-                    const int db = bs[i][j] + br[j] - ws[i];  // wave-uniform
-                    const TC T = os[i][j] + orr[j];
-                    int kk;
-                    if constexpr (!NEG && sizeof(TC) == 4) {
-                        f = __builtin_amdgcn_fractf(T);
-                        kk = floor_int(T) + db;
-                    } else {
-                        const TC fl = floor(T);
-                        f = T - fl;
-                        kk = (int)fl + db;
-                        if constexpr (NEG) {
-                            // the reference truncates toward zero: a time in
-                            // (-1, 0) reads sample 0 with a negative weight
-                            if (kk + ws[i] == -1 && f > TC(0)) {
-                                kk += 1;
-                                f -= TC(1);
-                            }
-                        }
-                    }
-                    return kk;
-                };
-                unsigned kmax = 0;
-#pragma unroll
-                for (int i = 0; i < JW; ++i) {
-#pragma unroll
-                    for (int j = 0; j < NP; ++j) {
-                        TC f;
-                        const unsigned kk = (unsigned)window_index(i, j, f);
-                        kmax = max(kmax, kk);
-                        // outside the window: position kWin - 1, the zero pair
-                        const unsigned kc = min(kk, (unsigned)(kWin - 1));
-                        TD wgt = TD(1);
-                        if constexpr (DYN) {
-                            wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
-                            if (pt[j].iz < izr[i])
-                                wgt = TD(0);
-                        }
-#pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                            const TP q = w[k][i][kc];
-                            TD v = fma((TD)f, q.v[1], q.v[0]);
-                            if constexpr (DYN)
-                                v *= wgt;
-                            acc[k][j] += v;
-                        }
-                    }
-                }
-                if (__ballot(kmax > (unsigned)(kWin - 2)) != 0) {
-                    // the samples outside the windows, from global memory
-#pragma unroll
-                    for (int i = 0; i < JW; ++i) {
-#pragma unroll
-                        for (int j = 0; j < NP; ++j) {
-                            TC fw;
-                            if ((unsigned)window_index(i, j, fw) > (unsigned)(kWin - 2)) {
-                                const TC T = os[i][j] + orr[j];
-                                const TC fl = floor(T);
-                                TC f = T - fl;
-                                int it = (int)fl + bs[i][j] + br[j];
-                                if constexpr (NEG) {
-                                    if (it == -1 && f > TC(0)) {
-                                        it = 0;
-                                        f -= TC(1);
-                                    }
+                        TC fw;
+                        if ((unsigned)window_index(i, j, fw) > (unsigned)(kWin - 2)) {
+                            const TC T = os[i][j] + orr[j];
+                            const TC fl = floor(T);
+                            TC f = T - fl;
+                            int it = (int)fl + bs[i][j] + br[j];
+                            if constexpr (NEG) {
+                                if (it == -1 && f > TC(0)) {
+                                    it = 0;
+                                    f -= TC(1);
                                 }
-                                if ((unsigned)it < ntm1) {
-                                    TD wgt = TD(1);
-                                    if constexpr (DYN) {
-                                        wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
-                                        if (pt[j].iz < izr[i])
-                                            wgt = TD(0);
-                                    }
+                            }
+                            if ((unsigned)it < ntm1) {
+                                TD wgt = TD(1);
+                                if constexpr (DYN) {
+                                    wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
+                                    if (pt[j].iz < izr[i])
+                                        wgt = TD(0);
+                                }
 #pragma unroll
-                                    for (int k = 0; k < K; ++k) {
-                                        const TD *q = dblk + k * dstride + (soff[i] + r * a.nt + it);
-                                        const TD g0 = q[0], g1 = q[1];
-                                        TD v = fma((TD)f, g1 - g0, g0);
-                                        if constexpr (DYN)
-                                            v *= wgt;
-                                        acc[k][j] += v;
-                                    }
+                                for (int k = 0; k < K; ++k) {
+                                    const TD *q = dblk + k * dstride + (soff[i] + r * a.nt + it);
+                                    const TD g0 = q[0], g1 = q[1];
+                                    TD v = fma((TD)f, g1 - g0, g0);
+                                    if constexpr (DYN)
+                                        v *= wgt;
+                                    acc[k][j] += v;
                                 }
                             }
                         }
                     }
                 }
             }
+        }
 #undef KX_STAGE
+    };
+    if (active) {
+        for (int s0 = 0; s0 < a.ns; s0 += SB) {
+            // slots i with a source s0 + part + P i < ns
+            const int nj = (a.ns - s0 - part + P - 1) / P;
+            if (nj > JW / 2)
+                source_block(IntC<JW>{}, s0);
+            else if (nj > 0)
+                source_block(IntC<(JW > 1 ? JW / 2 : 1)>{}, s0);
         }
     }
     if constexpr (P > 1) {
