@@ -353,7 +353,8 @@ __device__ __forceinline__ void spread_load(const KArgs &a, const TileCursor &c,
     const int orig = (c.tyi * a.ty * a.nx + c.txi * a.tx) * a.nz + c.tzi * a.tz;
     // Row pointers are wave-uniform and the lane offsets 32-bit unsigned, so
     // every load takes the scalar-base form (one VALU op of addressing).
-    L.mv = valid ? (m + orig)[(unsigned)rel] : TD(0);
+    const TD mval = (m + orig)[valid ? (unsigned)rel : 0u];
+    L.mv = valid ? mval : TD(0);
     const size_t ts = (size_t)t * a.ns + s;        // (tile, source) row
     const size_t tr = (size_t)t * a.nr + r0;       // (tile, first receiver) row
     L.os = (reinterpret_cast<const typename SrcTab<TT>::type *>(a.off_s) + ts * kWave)[(unsigned)lane];
@@ -727,22 +728,27 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     // The next tile is loaded while this one is accumulated, in one of
     // three loop forms, chosen by measurement (3D 128x128x120 kinematic,
     // rotated order; 2D 601x300 dynamic, natural order):
-    //   kFormAhead (float16 tables): loads only ever fill nxt, copied to cur
-    //     at the top of the loop, so that the wait for a tile's loads falls
-    //     a whole accumulation after they were issued (5% faster in 3D);
     //   kFormPair (float32 kinematic): two tile buffers taken in turn, the
-    //     loop unrolled twice, no copies (2% faster in 3D; 2-6% slower with
-    //     dynamic weights or float16 tables);
+    //     loop unrolled twice, no copies (float32 tables 2%, float16 tables
+    //     1.5% faster in 3D than the next best form);
+    //   kFormAhead (float16 tables otherwise): loads only ever fill nxt,
+    //     copied to cur at the top of the loop, so that the wait for a
+    //     tile's loads falls a whole accumulation after they were issued;
     //   kFormEnd (otherwise): the first tile is loaded into cur and nxt
-    //     copied over it at the end, where the compiler waits for the
-    //     prefetch just issued.  With float32 tables the first form is no
-    //     faster: its waves drift apart and the workgroups of one receiver
-    //     block (one per source) stop sharing its rows in L2 (102 against
-    //     57 GB from HBM per launch in 3D).
+    //     copied over it at the end.  With float32 tables the ahead form is
+    //     no faster: its waves drift apart and the workgroups of one
+    //     receiver block (one per source) stop sharing its rows in L2 (102
+    //     against 57 GB from HBM per launch in 3D).
+    // Every iteration issues the same loads: past the last tile the
+    // current one is loaded again, unused, and a point past the grid's
+    // edge reads the tile's first image value.  With a load under a branch
+    // the compiler cannot count the loads in flight on every path and
+    // waits for the prefetch just issued before each receiver's atomics
+    // (4% slower in 3D, 3% in 2D).
     constexpr int kFormAhead = 0, kFormPair = 1, kFormEnd = 2;
-    constexpr int kForm = std::is_same<TT, __half>::value
-                              ? kFormAhead
-                              : (!DYN && sizeof(TD) == 4 ? kFormPair : kFormEnd);
+    constexpr int kForm = !DYN && sizeof(TD) == 4
+                              ? kFormPair
+                              : (std::is_same<TT, __half>::value ? kFormAhead : kFormEnd);
     TileCursor c;
     c.start(a, wave);
     if constexpr (kForm == kFormPair) {
@@ -752,18 +758,16 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
         while (c.t < a.ntiles) {
             TileCursor cn = c;
             cn.advance(a, nw);
-            if (cn.t < a.ntiles)
-                spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
-                                                  tb);
+            spread_load<TD, TT, DYN, ORD, RB>(a, cn.t < a.ntiles ? cn : c, lane, rel, lz, lx, ly,
+                                              s, r0, nrl, m, vz, tb);
             spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS, SEG>(
                 ta, c.tzi * a.tz + lz, e, lrow, izm, ntm1, lds, a.t0, (unsigned)a.tlen);
             c = cn;
             if (c.t >= a.ntiles)
                 break;
             cn.advance(a, nw);
-            if (cn.t < a.ntiles)
-                spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
-                                                  ta);
+            spread_load<TD, TT, DYN, ORD, RB>(a, cn.t < a.ntiles ? cn : c, lane, rel, lz, lx, ly,
+                                              s, r0, nrl, m, vz, ta);
             spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS, SEG>(
                 tb, c.tzi * a.tz + lz, e, lrow, izm, ntm1, lds, a.t0, (unsigned)a.tlen);
             c = cn;
@@ -779,9 +783,8 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
                 cur = nxt;
             TileCursor cn = c;
             cn.advance(a, nw);
-            if (cn.t < a.ntiles)
-                spread_load<TD, TT, DYN, ORD, RB>(a, cn, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
-                                                  nxt);
+            spread_load<TD, TT, DYN, ORD, RB>(a, cn.t < a.ntiles ? cn : c, lane, rel, lz, lx, ly,
+                                              s, r0, nrl, m, vz, nxt);
             spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS, SEG>(
                 cur, c.tzi * a.tz + lz, e, lrow, izm, ntm1, lds, a.t0, (unsigned)a.tlen);
             if constexpr (!kAhead)
@@ -849,8 +852,11 @@ __device__ __forceinline__ void models_load(const KArgs &a, const TileCursor &c,
     const bool valid = (iz < a.nz) & (ix < a.nx) & (iy < a.ny);
     const int orig = (c.tyi * a.ty * a.nx + c.txi * a.tx) * a.nz + c.tzi * a.tz;
 #pragma unroll
-    for (int j = 0; j < K; ++j)
-        L.mv[j] = valid ? (m + j * ni + orig)[(unsigned)rel] : 0.f;
+    for (int j = 0; j < K; ++j) {
+        // unconditional loads, as in spread_load
+        const float mval = (m + j * ni + orig)[valid ? (unsigned)rel : 0u];
+        L.mv[j] = valid ? mval : 0.f;
+    }
     const size_t ts = (size_t)t * a.ns + s;
     const size_t tr = (size_t)t * a.nr + r;
     L.os = (reinterpret_cast<const typename SrcTab<TT>::type *>(a.off_s) + ts * kWave)[(unsigned)lane];
@@ -959,8 +965,8 @@ __device__ __forceinline__ void spread_models_body(const KArgs &a, const float *
     while (c.t < a.ntiles) {
         TileCursor cn = c;
         cn.advance(a, nw);
-        if (cn.t < a.ntiles)
-            models_load<TT, DYN, K>(a, cn, lane, rel, lz, lx, ly, s, r, m, ni, vz, nxt);
+        models_load<TT, DYN, K>(a, cn.t < a.ntiles ? cn : c, lane, rel, lz, lx, ly, s, r, m, ni,
+                                vz, nxt);
         models_accumulate<TT, DYN, NEG, FIX, K>(cur, c.tzi * a.tz + lz, e, izm, ntm1, a.nt,
                                                 accd, acci);
         cur = nxt;
