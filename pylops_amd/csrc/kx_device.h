@@ -725,33 +725,27 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
     }
     const unsigned ntm1 = (unsigned)(a.nt - 1);
 
-    // The next tile is loaded while this one is accumulated, in one of
-    // three loop forms, chosen by measurement (3D 128x128x120 kinematic,
-    // rotated order; 2D 601x300 dynamic, natural order):
-    //   kFormPair (float32 kinematic): two tile buffers taken in turn, the
-    //     loop unrolled twice, no copies (float32 tables 2%, float16 tables
-    //     1.5% faster in 3D than the next best form);
-    //   kFormAhead (float16 tables otherwise): loads only ever fill nxt,
-    //     copied to cur at the top of the loop, so that the wait for a
-    //     tile's loads falls a whole accumulation after they were issued;
-    //   kFormEnd (otherwise): the first tile is loaded into cur and nxt
-    //     copied over it at the end.  With float32 tables the ahead form is
-    //     no faster: its waves drift apart and the workgroups of one
-    //     receiver block (one per source) stop sharing its rows in L2 (102
-    //     against 57 GB from HBM per launch in 3D).
+    // The next tile is loaded while this one is accumulated.  Float32
+    // operators take two tile buffers in turn, the loop unrolled twice, no
+    // copies; float64 operators (more registers per tile) load the next
+    // tile into nxt and copy it over cur at the end of the loop.  Measured
+    // against those copies, and against loads that only ever fill nxt with
+    // a copy to cur at the top of the loop: 3D 128x128x120 kinematic 2%
+    // (float32 tables) and 1.5% (float16 tables) faster, 2D 601x300 dynamic
+    // 1% and 2% faster.  The loads-ahead form also lets the waves of a
+    // workgroup drift apart, so that the workgroups of one receiver block
+    // (one per source) stop sharing its rows in L2 (102 against 57 GB from
+    // HBM per launch in 3D).
     // Every iteration issues the same loads: past the last tile the
     // current one is loaded again, unused, and a point past the grid's
     // edge reads the tile's first image value.  With a load under a branch
     // the compiler cannot count the loads in flight on every path and
     // waits for the prefetch just issued before each receiver's atomics
     // (4% slower in 3D, 3% in 2D).
-    constexpr int kFormAhead = 0, kFormPair = 1, kFormEnd = 2;
-    constexpr int kForm = !DYN && sizeof(TD) == 4
-                              ? kFormPair
-                              : (std::is_same<TT, __half>::value ? kFormAhead : kFormEnd);
+    constexpr bool kPair = sizeof(TD) == 4;
     TileCursor c;
     c.start(a, wave);
-    if constexpr (kForm == kFormPair) {
+    if constexpr (kPair) {
         SpreadTile<TD, TT, ORD, RB> ta, tb;
         if (c.t < a.ntiles)
             spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, ta);
@@ -773,22 +767,17 @@ __device__ __forceinline__ void spread_body(const KArgs &a, const TD *__restrict
             c = cn;
         }
     } else {
-        constexpr bool kAhead = kForm == kFormAhead;
         SpreadTile<TD, TT, ORD, RB> cur, nxt;
         if (c.t < a.ntiles)
-            spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz,
-                                              kAhead ? nxt : cur);
+            spread_load<TD, TT, DYN, ORD, RB>(a, c, lane, rel, lz, lx, ly, s, r0, nrl, m, vz, cur);
         while (c.t < a.ntiles) {
-            if constexpr (kAhead)
-                cur = nxt;
             TileCursor cn = c;
             cn.advance(a, nw);
             spread_load<TD, TT, DYN, ORD, RB>(a, cn.t < a.ntiles ? cn : c, lane, rel, lz, lx, ly,
                                               s, r0, nrl, m, vz, nxt);
             spread_accumulate<TD, TT, DYN, NEG, ORD, ACC, RB, XS, SEG>(
                 cur, c.tzi * a.tz + lz, e, lrow, izm, ntm1, lds, a.t0, (unsigned)a.tlen);
-            if constexpr (!kAhead)
-                cur = nxt;
+            cur = nxt;
             c = cn;
         }
     }
