@@ -126,6 +126,20 @@ template <> __device__ __forceinline__ float to_tab<float>(double v) { return (f
 template <> __device__ __forceinline__ double to_tab<double>(double v) { return v; }
 template <> __device__ __forceinline__ __half to_tab<__half>(double v) { return __float2half((float)v); }
 
+// Offset of an entry that no trace sample can take (marked NaN by the
+// caller: its time lies past the last sample, or before -1, whatever the
+// other side's time): finite, exact, and so large that every sum with the
+// other side's offset and both bases (each within +-2^27) falls past the
+// trace while two such offsets still add up to less than 2^31, so the
+// kernels' bounds tests skip it without any conversion saturating.  Such
+// entries take no part in their tile's base, which keeps the others'
+// offsets small.
+template <typename TT> __device__ __forceinline__ TT unreached_offset() { return TT(0x1p29); }
+template <> __device__ __forceinline__ __half unreached_offset<__half>()
+{
+    return __float2half(65504.0f);  // largest finite float16
+}
+
 template <typename TT>
 __global__ __launch_bounds__(kBlock) void encode_kernel(KArgs a, const double *__restrict__ tabT,
                                                         int n, int64_t ni, double scale,
@@ -142,17 +156,20 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(KArgs a, const double *_
     p.iz = min(p.iz, a.nz - 1);
     p.ix = min(p.ix, a.nx - 1);
     p.iy = min(p.iy, a.ny - 1);
-    const double q = tabT[(size_t)j * ni + point_index(a, p)] * scale;
-    double mn = q, mx = q;
+    const double q = tabT[(size_t)j * ni + point_index(a, p)] * scale;  // NaN: unreached
+    double mn = q, mx = q;  // fmin / fmax pass over NaN
 #pragma unroll
     for (int o = kWave / 2; o >= 1; o >>= 1) {
         mn = fmin(mn, __shfl_xor(mn, o, kWave));
         mx = fmax(mx, __shfl_xor(mx, o, kWave));
     }
+    // NaN when no entry is reached, and then -2^27.  Bases lie in +-2^27
+    // (134M samples, far past any trace), so that the adjoint's sums of the
+    // source and receiver bases of four tiles stay inside 32 bits
     double b = floor(0.5 * (mn + mx) + 0.5);
-    b = fmin(fmax(b, -536870912.0), 536870912.0);
+    b = fmin(fmax(b, -134217728.0), 134217728.0);
     const size_t o = (size_t)t * n + j;  // tile-major output
-    off[o * kWave + lane] = to_tab<TT>(q - b);
+    off[o * kWave + lane] = q == q ? to_tab<TT>(q - b) : unreached_offset<TT>();
     if (lane == 0)
         base[o] = (int)b;
 }

@@ -8,6 +8,7 @@ byot, K:351-360), amplitudes K:316-330, angles K:364-393, six/rix
 K:289-301, apertures K:414-439.
 """
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -255,12 +256,22 @@ def make_geom_plan(nx, ny, nz, tile):
     return p
 
 
-def encode(plan, tab, scale, fmt):
+def encode(plan, tab, scale, fmt, lo=-math.inf, hi=math.inf):
     """tab: float64 device tensor (ni, n) in reference order -> (off, base)
-    device tensors in the tiled layout, offsets in samples (tab*scale)."""
+    device tensors in the tiled layout, offsets in samples (tab*scale).
+
+    Entries outside [lo, hi) samples are ones that no trace sample can take
+    (the caller derives the bounds from the other side's extreme times);
+    they are passed to the encoder as NaN, which gives them an offset far
+    outside every trace and leaves them out of their tile's base, so that
+    unreachable points (huge times) cost the others no precision."""
     n = tab.shape[1]
     if not bool(torch.isfinite(tab).all()):
         raise ValueError("traveltime table contains non-finite values")
+    q = tab * scale
+    out = (q < lo) | (q >= hi)
+    if bool(out.any()):
+        tab = torch.where(out, torch.full_like(tab, math.nan), tab)
     tabT = tab.t().contiguous()
     tdt = {_lib.KX_F32: torch.float32, _lib.KX_F16: torch.float16,
            _lib.KX_F64: torch.float64}[fmt]

@@ -359,12 +359,35 @@ class Kirchhoff(LinearOperator):
                              f"{tuple(tr.shape)} do not match (ni, n) = ({self.ni}, {ns}) / "
                              f"({self.ni}, {nr})")
         neg = bool((ts.min() + tr.min()) < 0)
+        # Entries that no trace sample can take, whatever the other side's
+        # time: a sum at or past sample nt - 1, or at or before -1 (the
+        # reference's bounds tests, K:743-751), kept 2 samples clear of
+        # either edge.  They are encoded apart (T.encode), so that huge
+        # times standing for unreachable points cost their tile's other
+        # points no precision.
+        s_min, s_max = float(ts.min()) * inv_dt, float(ts.max()) * inv_dt
+        r_min, r_max = float(tr.min()) * inv_dt, float(tr.max()) * inv_dt
+        nt = self.nt
+        lo_s, hi_s = -3.0 - r_max, nt + 1.0 - r_min
+        lo_r, hi_r = -3.0 - s_max, nt + 1.0 - s_min
+        if fmt == _lib.KX_F16:
+            # a float16 offset reaches only 65504 samples past its base
+            q = tr * inv_dt
+            out = (q < lo_r) | (q >= hi_r)
+            if bool(out.any()):
+                reached = float(q[~out].min()) if bool((~out).any()) else 0.0
+                if nt + 4.0 > 65504.0 + min(0.0, s_min) + min(0.0, reached):
+                    raise NotImplementedError(
+                        f"float16 tables cannot mark the unreachable receiver times of "
+                        f"nt={nt} traces; use float32 tables")
+            del q, out
         # float16 offsets on the receiver side only: the source tables (a
         # few rows per tile) stay float32, so that a traveltime sum carries
         # one float16 rounding
         keep["off_s"], keep["base_s"] = T.encode(plan, ts, inv_dt,
-                                                 _lib.KX_F32 if fmt == _lib.KX_F16 else fmt)
-        keep["off_r"], keep["base_r"] = T.encode(plan, tr, inv_dt, fmt)
+                                                 _lib.KX_F32 if fmt == _lib.KX_F16 else fmt,
+                                                 lo_s, hi_s)
+        keep["off_r"], keep["base_r"] = T.encode(plan, tr, inv_dt, fmt, lo_r, hi_r)
         del ts, tr
         wfmt = _lib.KX_F64 if self.dtype == np.float64 else _lib.KX_F32
         if self.dynamic:

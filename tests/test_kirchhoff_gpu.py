@@ -726,3 +726,39 @@ def test_long_traces_in_time_segments(dynamic):
     orc64 = O.KirchhoffOracle(**kw64)
     assert G.relerr(np.asarray(Op64 @ m.astype(np.float64)),
                     orc64.matvec(m.astype(np.float64))) < 1e-10
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_unreachable_traveltimes_contribute_nothing(dtype):
+    """Traveltimes far past every trace (1e6 s: unreachable points, whose
+    tile bases the table encoding clamps to 2^29 samples) fall outside the
+    trace and add nothing, as the reference's bounds test skips them
+    (K:743-751, 754-781).  One source and one receiver are unreachable
+    everywhere, so tile groups whose source and receiver bases are both
+    clamped meet in the adjoint, and a band of depths is unreachable for
+    every source; every adjoint tile-group width is checked."""
+    Op0 = _K(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+             wavcenter=G.wavc, mode="analytic")
+    ts = np.array(Op0.trav_srcs, dtype=np.float64)
+    tr = np.array(Op0.trav_recs, dtype=np.float64)
+    ts[:, 0] = 1e6
+    tr[:, -1] = 1e6
+    band = (np.arange(ts.shape[0]) % len(G.z)) >= 12
+    ts[band, :] = 1e6
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="byot", trav=(ts, tr), dtype=dtype)
+    Op = _K(**kw)
+    orc = O.KirchhoffOracle(**kw)
+    rng = np.random.default_rng(12)
+    m = rng.standard_normal(Op.shape[1]).astype(dtype)
+    d = rng.standard_normal(Op.shape[0]).astype(dtype)
+    tol = 1e-5 if dtype == "float32" else 1e-10
+    want_s = orc.spread(m)
+    want_g = orc.gather(d)
+    assert np.linalg.norm(want_s) > 0 and np.linalg.norm(want_g) > 0
+    assert G.relerr(np.asarray(Op.spread(m)), want_s) < tol
+    for gt in (1, 2, 4):
+        for gs in (1, 4):
+            Op.plan.gtiles, Op.plan.gsplit = gt, gs
+            assert G.relerr(np.asarray(Op.gather(d)), want_g) < tol, (gt, gs)
+    Op.plan.gtiles = Op.plan.gsplit = 0
