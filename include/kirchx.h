@@ -143,7 +143,12 @@ int64_t kx_workspace_bytes(const kx_plan *p);
 /* Table layout: reads a [n][ni] float64 table tabT (reference table
  * transposed), multiplies by `scale` (1/dt), and writes off/base in the
  * tiled layout; fmt = KX_F32 / KX_F16 / KX_F64.  Padding lanes replicate
- * the nearest grid point. */
+ * the nearest grid point.  A NaN entry marks a time that no trace sample
+ * can take whatever the other side's time (past sample nt - 1 or before
+ * -1 even with the other side's extreme): it gets an offset of 2^29
+ * samples (65504 with KX_F16) and no part in its tile's base, so the
+ * kernels skip it and its tile's other entries keep small offsets.  Bases
+ * are clamped to +-2^27 samples (-2^27 for a tile of NaN entries only). */
 int kx_encode_table(const kx_plan *geom, const double *tabT, int32_t n,
                     double scale, int32_t fmt, void *off, int32_t *base,
                     void *stream);

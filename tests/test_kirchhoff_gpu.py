@@ -762,3 +762,42 @@ def test_unreachable_traveltimes_contribute_nothing(dtype):
             Op.plan.gtiles, Op.plan.gsplit = gt, gs
             assert G.relerr(np.asarray(Op.gather(d)), want_g) < tol, (gt, gs)
     Op.plan.gtiles = Op.plan.gsplit = 0
+
+
+def test_unreachable_traveltimes_float16_tables():
+    """The same unreachable times with float16 receiver offsets (65504
+    samples for an unreachable entry): the unreachable traces are exactly
+    zero, and the other entries lose no precision to the marking (the
+    error against the oracle stays within 1.5x that of float16 tables of
+    the unmodified times, ~1e-3 on this grid: offsets of tens of samples
+    at 2^-6 sample resolution); traces too long for that offset to clear
+    them are refused."""
+    Op0 = _K(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+             wavcenter=G.wavc, mode="analytic")
+    ts0 = np.array(Op0.trav_srcs, dtype=np.float64)
+    tr0 = np.array(Op0.trav_recs, dtype=np.float64)
+    ts, tr = ts0.copy(), tr0.copy()
+    ts[:, 0] = 1e6
+    tr[:, -1] = 1e6
+    tr[(np.arange(tr.shape[0]) % len(G.z)) >= 12, 1] = 1e6
+    base = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+                wavcenter=G.wavc, mode="byot", dtype="float32")
+    rng = np.random.default_rng(13)
+    ns, nr, nt = G.s2d.shape[1], G.r2d.shape[1], len(G.t)
+    m = rng.standard_normal(len(G.x) * len(G.z)).astype(np.float32)
+    d = rng.standard_normal(ns * nr * nt).astype(np.float32)
+    errs = {}
+    for name, tabs in (("plain", (ts0, tr0)), ("unreachable", (ts, tr))):
+        kw = dict(base, trav=tabs)
+        Op = _K(**kw, tabledtype="float16")
+        orc = O.KirchhoffOracle(**kw)
+        fwd = np.asarray(Op.spread(m))
+        errs[name] = (G.relerr(fwd, orc.spread(m)), G.relerr(np.asarray(Op.gather(d)), orc.gather(d)))
+    f = fwd.reshape(ns, nr, nt)
+    assert not f[0].any() and not f[:, -1].any()
+    for k in range(2):
+        assert errs["unreachable"][k] < 1.5 * errs["plain"][k] + 1e-4, errs
+    kw = dict(base, trav=(ts, tr))
+    tlong = np.arange(70000) * float(G.t[1] - G.t[0])
+    with pytest.raises(NotImplementedError, match="float16"):
+        _K(**dict(kw, t=tlong), tabledtype="float16")
