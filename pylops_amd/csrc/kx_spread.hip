@@ -245,7 +245,7 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
     SpreadScale *sc = reinterpret_cast<SpreadScale *>(p->aux);
     int side = p->sside;
     if (side == KX_SIDE_AUTO)
-        side = (!p->dynamic && p->ny == 1 && p->ns < p->nr) ? KX_SIDE_SOURCES : KX_SIDE_RECEIVERS;
+        side = (p->ny == 1 && p->ns < p->nr) ? KX_SIDE_SOURCES : KX_SIDE_RECEIVERS;
     // (float16 plans keep the two sides' offsets in different types and
     // block receivers)
     const bool xs = sizeof(TD) == 4 && side == KX_SIDE_SOURCES && p->tab_fmt != KX_F16;
@@ -266,10 +266,12 @@ int launch_spread_d(const kx_plan *p, KArgs a, const void *m, void *y, hipStream
     // transposed positions: T = os + or and the integer bases add in either
     // order, and the dynamic kernel (XS) forms the source-side products per
     // step, so the traces are bitwise those of blocks of receivers.
-    // Automatic: a 2D kinematic plan blocks the side with fewer members,
-    // whose rows then serve many workgroups through L2 (13% faster on a
-    // 61 x 301 geometry, either way round); 3D and dynamic plans block
-    // receivers (measured no faster otherwise).
+    // Automatic: a 2D plan blocks the side with fewer members, whose rows
+    // then serve many workgroups through L2 (on a 61 x 301 geometry 13%
+    // faster kinematic, either way round, and 6% with dynamic weights: 3.68
+    // against 3.91 ms for the forward with its wavelet); 3D plans block
+    // receivers (measured slower otherwise: 128x128x120 with 25 x 400, 24.2
+    // against 22.8 ms for the natural order, 17.4 against 15.8 ms rotated).
     if (seg)
         return p->dynamic ? (p->neg_trav ? launch_spread_seg<TD, TT, true, true>(a, m, y, sc, st)
                                          : launch_spread_seg<TD, TT, true, false>(a, m, y, sc, st))
