@@ -268,8 +268,8 @@ def encode(plan, tab, scale, fmt, lo=-math.inf, hi=math.inf):
     n = tab.shape[1]
     if not bool(torch.isfinite(tab).all()):
         raise ValueError("traveltime table contains non-finite values")
-    q = tab * scale
-    out = (q < lo) | (q >= hi)
+    # scale (1/dt) > 0: compare the table itself, no scaled copy
+    out = (tab < lo / scale) | (tab >= hi / scale)
     if bool(out.any()):
         tab = torch.where(out, torch.full_like(tab, math.nan), tab)
     tabT = tab.t().contiguous()

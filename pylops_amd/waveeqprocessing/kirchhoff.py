@@ -372,15 +372,16 @@ class Kirchhoff(LinearOperator):
         lo_r, hi_r = -3.0 - s_max, nt + 1.0 - s_min
         if fmt == _lib.KX_F16:
             # a float16 offset reaches only 65504 samples past its base
-            q = tr * inv_dt
-            out = (q < lo_r) | (q >= hi_r)
+            out = (tr < lo_r * self.dt) | (tr >= hi_r * self.dt)
             if bool(out.any()):
-                reached = float(q[~out].min()) if bool((~out).any()) else 0.0
+                keep_r = tr[~out]
+                reached = float(keep_r.min()) * inv_dt if keep_r.numel() else 0.0
                 if nt + 4.0 > 65504.0 + min(0.0, s_min) + min(0.0, reached):
                     raise NotImplementedError(
                         f"float16 tables cannot mark the unreachable receiver times of "
                         f"nt={nt} traces; use float32 tables")
-            del q, out
+                del keep_r
+            del out
         # float16 offsets on the receiver side only: the source tables (a
         # few rows per tile) stay float32, so that a traveltime sum carries
         # one float16 rounding
