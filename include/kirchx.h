@@ -63,6 +63,9 @@ extern "C" {
 #define KX_SIDE_RECEIVERS 1
 #define KX_SIDE_SOURCES 2
 
+/* plan.dynamic value for dynamic plans whose aperture gates are all 0 */
+#define KX_DYNAMIC_NOGATE 2
+
 /* element types */
 #define KX_F32 0
 #define KX_F64 1
@@ -94,7 +97,10 @@ typedef struct kx_plan {
     int32_t abi_version; /* KX_ABI_VERSION */
     int32_t dtype;       /* KX_F32 or KX_F64: model, data and accumulation */
     int32_t tab_fmt;     /* KX_F32, KX_F16 (dtype F32 only) or KX_F64 */
-    int32_t dynamic;     /* 1: amplitude/obliquity/aperture weights */
+    int32_t dynamic;     /* 1: amplitude/obliquity/aperture weights;
+                            KX_DYNAMIC_NOGATE (2): the same with every
+                            izmin entry 0 (no offset-aperture gate), which
+                            lets the adjoint skip the gate test */
     int32_t ns, nr, nt;  /* sources, receivers, time samples */
     int32_t nx, ny, nz;  /* image grid, ny = 1 in 2D */
     int32_t tz, tx, ty;  /* tile shape, tz*tx*ty == 64 */

@@ -17,6 +17,7 @@ KX_AUX_BYTES = 4096
 KX_F32, KX_F64, KX_F16 = 0, 1, 2
 KX_ORDER_AUTO, KX_ORDER_NATURAL, KX_ORDER_ROTATED = 0, 1, 2
 KX_SIDE_AUTO, KX_SIDE_RECEIVERS, KX_SIDE_SOURCES = 0, 1, 2
+KX_DYNAMIC_NOGATE = 2
 KX_EINVAL, KX_ESHAPE, KX_ETYPE, KX_EABI, KX_ELDS = -1, -2, -3, -4, -5
 
 EXPORTS = (

@@ -436,7 +436,7 @@ int check_batch(const kx_plan *plans, int nprob)
         int rc = kx_plan_check(&a);
         if (rc)
             return rc;
-        if (a.dtype != q.dtype || a.tab_fmt != q.tab_fmt || a.dynamic != q.dynamic ||
+        if (a.dtype != q.dtype || a.tab_fmt != q.tab_fmt || (a.dynamic != 0) != (q.dynamic != 0) ||
             a.neg_trav != q.neg_trav)
             return bfail(KX_EINVAL,
                          "batch: problem %d differs from problem 0 in dtype, table format, "

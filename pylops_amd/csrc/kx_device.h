@@ -1170,7 +1170,8 @@ inline void set_tile_groups(KArgs &a, int np)
 // positions; each model has its own staged windows and sums, with the
 // single-model operations in the single-model order, so each image is
 // bitwise the one K = 1 gives.
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP, int K = 1>
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP, int K = 1,
+          bool GATE = true>
 __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restrict__ d,
                                              TD *__restrict__ m, int L, int64_t dstride = 0,
                                              int64_t mstride = 0)
@@ -1281,7 +1282,7 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
         _Pragma("unroll") for (int k = 0; k < K; ++k)                                         \
             sv[k][i] = dblk[k * dstride + (unsigned)(soff[i] + (R) * a.nt + ec)];              \
         svok[i] = (unsigned)e < ntm1 && lane < kWin - 1;                                       \
-        if constexpr (DYN)                                                                     \
+        if constexpr (DYN && GATE)                                                             \
             izn[i] = a.izmin[(unsigned)(izoff[i] + (R))];                                      \
     }
         // receiver side of the round ahead; receiver bases two rounds
@@ -1392,7 +1393,7 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                     TD wgt = TD(1);
                     if constexpr (DYN) {
                         wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
-                        if (pt[j].iz < izr[i])
+                        if (GATE && pt[j].iz < izr[i])
                             wgt = TD(0);
                     }
 #pragma unroll
@@ -1427,7 +1428,7 @@ __device__ __forceinline__ void gather_tiles(const KArgs &a, const TD *__restric
                                 TD wgt = TD(1);
                                 if constexpr (DYN) {
                                     wgt = ps[i][j] * pr[j] + qs[i][j] * qr[j];
-                                    if (pt[j].iz < izr[i])
+                                    if (GATE && pt[j].iz < izr[i])
                                         wgt = TD(0);
                                 }
 #pragma unroll

@@ -13,14 +13,16 @@
 namespace {
 
 // K models (K > 1: model k's traces at d + k*dstride, its image at
-// m + k*mstride); groups of NP tiles per wave
-template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP, int K>
+// m + k*mstride); groups of NP tiles per wave; GATE: apply the per-trace
+// aperture gate (dynamic plans whose gates are all 0 skip it)
+template <typename TD, typename TT, bool DYN, bool NEG, int JW, int P, int NP, int K,
+          bool GATE = true>
 __global__ __launch_bounds__(kGatherBlock) void gather_kernel(KArgs a, const TD *__restrict__ d,
                                                               TD *__restrict__ m,
                                                               int64_t dstride, int64_t mstride)
 {
-    gather_tiles<TD, TT, DYN, NEG, JW, P, NP, K>(a, d, m, xcd_remap(blockIdx.x, gridDim.x),
-                                                 dstride, mstride);
+    gather_tiles<TD, TT, DYN, NEG, JW, P, NP, K, GATE>(a, d, m, xcd_remap(blockIdx.x, gridDim.x),
+                                                       dstride, mstride);
 }
 
 // sources per wave and pass (measured): every pass re-reads the receiver
@@ -30,7 +32,7 @@ template <bool DYN, int NP, int K> struct GatherJW {
     static constexpr int v = (DYN && NP == 1 && K == 1) ? 8 : 4;
 };
 
-template <typename TD, typename TT, bool DYN, bool NEG, int P, int NP, int K>
+template <typename TD, typename TT, bool DYN, bool NEG, int P, int NP, int K, bool GATE = true>
 int launch_gather_p(KArgs a, const void *d, int64_t dstride, void *m, int64_t mstride,
                     hipStream_t st)
 {
@@ -40,7 +42,7 @@ int launch_gather_p(KArgs a, const void *d, int64_t dstride, void *m, int64_t ms
         return kx_failf(KX_ESHAPE, "gather: %d sources x nr x nt exceeds 2^31 samples", JW * P);
     set_tile_groups(a, NP);
     const int nblk = (a.ngroups + NG - 1) / NG;
-    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P, NP, K>), dim3(nblk),
+    hipLaunchKernelGGL((gather_kernel<TD, TT, DYN, NEG, JW, P, NP, K, GATE>), dim3(nblk),
                        dim3(kGatherBlock), 0, st, a, reinterpret_cast<const TD *>(d),
                        reinterpret_cast<TD *>(m), dstride, mstride);
     KX_LAUNCHED("gather");
@@ -102,21 +104,21 @@ void gather_shape(const KArgs &a, bool dyn, int k, int gsplit, int gtiles, int *
     *p_out = split_for(np);
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int NP, int K>
+template <typename TD, typename TT, bool DYN, bool NEG, int NP, int K, bool GATE = true>
 int launch_gather_np(const KArgs &a, int P, const void *d, void *m, hipStream_t st,
                      int64_t dstride, int64_t mstride)
 {
     switch (P) {
     case 1:
-        return launch_gather_p<TD, TT, DYN, NEG, 1, NP, K>(a, d, dstride, m, mstride, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 1, NP, K, GATE>(a, d, dstride, m, mstride, st);
     case 2:
-        return launch_gather_p<TD, TT, DYN, NEG, 2, NP, K>(a, d, dstride, m, mstride, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 2, NP, K, GATE>(a, d, dstride, m, mstride, st);
     default:
-        return launch_gather_p<TD, TT, DYN, NEG, 4, NP, K>(a, d, dstride, m, mstride, st);
+        return launch_gather_p<TD, TT, DYN, NEG, 4, NP, K, GATE>(a, d, dstride, m, mstride, st);
     }
 }
 
-template <typename TD, typename TT, bool DYN, bool NEG, int K = 1>
+template <typename TD, typename TT, bool DYN, bool NEG, int K = 1, bool GATE = true>
 int launch_gather_t(const KArgs &a, int gsplit, int gtiles, const void *d, void *m,
                     hipStream_t st, int64_t dstride = 0, int64_t mstride = 0)
 {
@@ -124,11 +126,11 @@ int launch_gather_t(const KArgs &a, int gsplit, int gtiles, const void *d, void 
     gather_shape(a, DYN, K, gsplit, gtiles, &np, &P);
     switch (np) {
     case 4:
-        return launch_gather_np<TD, TT, DYN, NEG, 4, K>(a, P, d, m, st, dstride, mstride);
+        return launch_gather_np<TD, TT, DYN, NEG, 4, K, GATE>(a, P, d, m, st, dstride, mstride);
     case 2:
-        return launch_gather_np<TD, TT, DYN, NEG, 2, K>(a, P, d, m, st, dstride, mstride);
+        return launch_gather_np<TD, TT, DYN, NEG, 2, K, GATE>(a, P, d, m, st, dstride, mstride);
     default:
-        return launch_gather_np<TD, TT, DYN, NEG, 1, K>(a, P, d, m, st, dstride, mstride);
+        return launch_gather_np<TD, TT, DYN, NEG, 1, K, GATE>(a, P, d, m, st, dstride, mstride);
     }
 }
 
@@ -136,6 +138,15 @@ template <typename TD, typename TT>
 int launch_gather_d(const kx_plan *p, const KArgs &a, const void *d, void *m, hipStream_t st)
 {
     const int g = p->gsplit, n = p->gtiles;
+    // dynamic == KX_DYNAMIC_NOGATE: every aperture gate is 0, and float32
+    // operators run a gather without the per-sample gate test (2 VALU
+    // instructions of about 20 per sample, and the gate loads); the image
+    // is bitwise that of the gated kernel
+    if constexpr (sizeof(TD) == 4)
+        if (p->dynamic == KX_DYNAMIC_NOGATE)
+            return p->neg_trav
+                       ? launch_gather_t<TD, TT, true, true, 1, false>(a, g, n, d, m, st)
+                       : launch_gather_t<TD, TT, true, false, 1, false>(a, g, n, d, m, st);
     if (p->dynamic)
         return p->neg_trav ? launch_gather_t<TD, TT, true, true>(a, g, n, d, m, st)
                            : launch_gather_t<TD, TT, true, false>(a, g, n, d, m, st);

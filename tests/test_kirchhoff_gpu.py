@@ -801,3 +801,27 @@ def test_unreachable_traveltimes_float16_tables():
     tlong = np.arange(70000) * float(G.t[1] - G.t[0])
     with pytest.raises(NotImplementedError, match="float16"):
         _K(**dict(kw, t=tlong), tabledtype="float16")
+
+
+@pytest.mark.parametrize("tabledtype", [None, "float16"])
+def test_dynamic_without_aperture_gates_skips_the_gate(tabledtype):
+    """A dynamic plan whose offset aperture gates no trace (the default
+    aperture) is flagged KX_DYNAMIC_NOGATE, and its adjoint runs without the
+    per-sample gate test: bitwise the image of the gated kernel, and the
+    oracle's; a plan with gates keeps them."""
+    from pylops_amd import _lib
+    kw = dict(z=G.z, x=G.x, t=G.t, srcs=G.s2d, recs=G.r2d, vel=G.v0, wav=G.wav,
+              wavcenter=G.wavc, mode="analytic", dynamic=True, dtype="float32",
+              angleaperture=60)
+    Op = _K(**kw, tabledtype=tabledtype)
+    assert Op.plan.dynamic == _lib.KX_DYNAMIC_NOGATE
+    d = np.random.default_rng(14).standard_normal(Op.shape[0]).astype(np.float32)
+    got = np.asarray(Op.gather(d))
+    Op.plan.dynamic = 1
+    gated = np.asarray(Op.gather(d))
+    Op.plan.dynamic = _lib.KX_DYNAMIC_NOGATE
+    np.testing.assert_array_equal(got, gated)
+    if tabledtype is None:
+        assert G.relerr(got, O.KirchhoffOracle(**kw).gather(d)) < 1e-5
+    Opg = _K(**dict(kw, aperture=2.0), tabledtype=tabledtype)
+    assert Opg.plan.dynamic == 1
